@@ -1,0 +1,332 @@
+/*
+ * solver_cpp_amd.h -- C-ABI of the MI355X-native spectral grid solver and its
+ * phenopowerlaw material-point update.
+ *
+ * Two libraries export exactly these symbols:
+ *   solver_cpp_amd/libsolver_cpp_amd.so   HIP/CDNA4 implementation (the product)
+ *   oracle/_build/liboracle.so            FP64 CPU restatement (test infrastructure only)
+ *
+ * Layouts follow the reference: second-order tensors are (3,3) column-major
+ * (element (i,j) at i+3j), fields are (3,3,Nx,Ny,Nz) column-major with the
+ * tensor components fastest and x fastest among cells, fourth-order tensors
+ * are (3,3,3,3) column-major.  Quaternions are (w,x,y,z).
+ *
+ * Every function that can fail returns 0 on success and a nonzero code
+ * otherwise; sc_last_error() returns the message of the last failure on the
+ * calling thread.
+ */
+#ifndef SOLVER_CPP_AMD_H
+#define SOLVER_CPP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Configuration                                                              */
+/* ------------------------------------------------------------------------- */
+
+/* Elastic + phenopowerlaw constants of one cF phase with N_sl = [12].
+ * Replaces the material.yaml phase entry parsed by material.f90:84-205,
+ * phase_mechanical_elastic.f90:74-104 and
+ * phase_mechanical_plastic_phenopowerlaw.f90:75-290. */
+typedef struct sc_material {
+  double C_11, C_12, C_44;        /* Pa */
+  double xi_0_sl, xi_inf_sl;      /* Pa */
+  double h_0_sl_sl;               /* Pa */
+  double n_sl, a_sl;              /* stress exponent, hardening exponent */
+  double dot_gamma_0_sl;          /* 1/s */
+  double h_sl_sl[7];              /* interaction values, cF types 1..7 */
+  double atol_xi, atol_gamma;     /* state tolerances (defaults 1.0, 1e-6) */
+} sc_material;
+
+/* Phase numerics, defaults of phase_mechanical.f90:280-294. */
+typedef struct sc_phase_numerics {
+  double r_cutback_min;           /* 1e-3 */
+  double r_cutback;               /* 0.25 */
+  double r_increase;              /* 1.5 */
+  double eps_rel_state;           /* 1e-6 */
+  int    N_iter_state_max;        /* 20 */
+  int    N_iter_Lp_max;           /* 40 */
+  double r_linesearch_Lp;         /* 0.5 */
+  double eps_rel_Lp;              /* 1e-6 */
+  double eps_abs_Lp;              /* 1e-8 */
+} sc_phase_numerics;
+
+/* Grid solver numerics, Config::NumGridMechanical / NumFFT / Numerics
+ * (src/grid/config.h:50-78). */
+typedef struct sc_grid_numerics {
+  int    itmin, itmax;            /* 1, 100 (examples/grid/numerics.yaml: 4, 40) */
+  int    update_gamma;            /* 0 */
+  double eps_abs_div, eps_rel_div;/* 1e-4, 5e-4 */
+  double eps_abs_P, eps_rel_P;    /* 1e3, 1e-3 */
+  int    divergence_correction;   /* 0 none, 1 size, 2 size+grid (default 2) */
+  int    derivative;              /* 0 continuous, 1 central diff, 2 FWBW */
+  int    stag_iter_max;           /* 10 */
+  int    max_cut_back;            /* 3 */
+  int    ngmres_m;                /* NGMRES history size (PETSc default 30) */
+} sc_grid_numerics;
+
+/* Arithmetic of the spectral stages. */
+enum {
+  SC_PRECISION_FP64  = 0,  /* everything FP64 (plain form) */
+  SC_PRECISION_MIXED = 1   /* FP32 FFT / Gamma / residual fields, FP64 material point,
+                              FP64 norms, FP64 finishing sweeps per increment */
+};
+
+typedef struct sc_rve_desc {
+  int           cells[3];
+  double        size[3];          /* m */
+  int           n_materials;
+  const double* orientations;     /* (4, n_materials), unit quaternions w,x,y,z */
+  const int*    material_at;      /* (Nx*Ny*Nz), 0-based material index, x fastest */
+  sc_material        material;
+  sc_phase_numerics  phase_num;
+  sc_grid_numerics   grid_num;
+  int           precision;        /* SC_PRECISION_* */
+} sc_rve_desc;
+
+/* One load step, Config::LoadStep (src/grid/config.h:80-96).
+ * Masks are 1 where the entry was 'x' (not prescribed). */
+enum { SC_DEF_F = 0, SC_DEF_DOT_F = 1, SC_DEF_L = 2 };
+enum { SC_STRESS_P = 0, SC_STRESS_DOT_P = 1 };
+typedef struct sc_loadstep {
+  int    deformation_type;        /* SC_DEF_* */
+  double deformation[9];          /* (3,3) column-major */
+  int    deformation_mask[9];
+  int    stress_type;             /* SC_STRESS_* */
+  double stress[9];
+  int    stress_mask[9];
+  double rot_q[4];                /* R, (w,x,y,z); identity = (1,0,0,0) */
+  double t;                       /* duration, s */
+  int    N;                       /* increments */
+  double r;                       /* geometric ratio (1 = constant) */
+  int    estimate_rate;           /* 1 */
+  int    f_out;                   /* result cadence */
+} sc_loadstep;
+
+/* Config::SolutionState (src/grid/config.h:35-40) plus counters. */
+typedef struct sc_solution {
+  int converged;
+  int stag_converged;
+  int terminally_ill;
+  int required_iterations;        /* total_iter at exit */
+  int residual_evaluations;       /* form_residual calls in this solve */
+  double err_div, err_BC;         /* last values */
+} sc_solution;
+
+/* Run statistics of one RVE context (for measurement). */
+typedef struct sc_stats {
+  int64_t residual_evaluations;   /* form_residual calls */
+  int64_t material_point_updates; /* N * residual evaluations */
+  int64_t increments;             /* converged increments */
+  int64_t cutbacks;
+  int64_t newton_iterations;      /* sum over points and evaluations (if counted) */
+  int64_t state_iterations;       /* sum over points and evaluations (if counted) */
+  double  seconds_total;          /* wall time inside sc_rve_* calls */
+} sc_stats;
+
+/* ------------------------------------------------------------------------- */
+/* Library                                                                    */
+/* ------------------------------------------------------------------------- */
+
+const char* sc_last_error(void);
+/* "hip-gfx950" for the product, "cpu-oracle" for the oracle. */
+const char* sc_backend_name(void);
+void sc_default_material(sc_material* m);         /* examples/grid/material.yaml:7-24 */
+void sc_default_phase_numerics(sc_phase_numerics* p);
+void sc_default_grid_numerics(sc_grid_numerics* g);
+
+/* ------------------------------------------------------------------------- */
+/* Device-resident RVE context (the spectral basic solver)                    */
+/* ------------------------------------------------------------------------- */
+
+typedef struct sc_rve sc_rve;
+
+/* DiscretizationGrid::init + Spectral::init + materialpoint init
+ * (discretization_grid.cpp:8-84, spectral.cpp:12-96, materialpoint.f90:68-156). */
+int  sc_rve_create(const sc_rve_desc* desc, sc_rve** out);
+void sc_rve_destroy(sc_rve* rve);
+
+/* MechSolverBasic::init, mech_solver_basic.cpp:19-108. */
+int sc_rve_init(sc_rve* rve);
+
+/* MechSolverBasic::forward, mech_solver_basic.cpp:163-220, followed by
+ * f_materialpoint_forward when cut_back == 0 (damask_grid.cpp:19-34). */
+int sc_rve_forward(sc_rve* rve, int cut_back, int guess, double dt, double dt_old,
+                   double t_remaining, const sc_loadstep* step);
+
+/* MechSolverBasic::calculate_solution, mech_solver_basic.cpp:110-161. */
+int sc_rve_solution(sc_rve* rve, sc_solution* out);
+
+/* MechSolverBasic::update_coords, mech_solver_basic.cpp:222-240
+ * (base_update_coords, mech_base.cpp:45-154). */
+int sc_rve_update_coords(sc_rve* rve);
+
+/* DamaskGrid::loop_over_loadcase, damask_grid.cpp:82-196, with stag_it_max
+ * and max_cut_back from the grid numerics.  Per-increment results are
+ * written to inc_iterations / inc_evaluations (length >= total increments,
+ * may be NULL).  Returns nonzero if an increment could not be solved. */
+int sc_rve_run_loadcase(sc_rve* rve, const sc_loadstep* steps, int n_steps,
+                        int* inc_iterations, int* inc_evaluations, int max_incs);
+
+/* One residual evaluation r = form_residual(F) on the current solver state
+ * (mech_solver_basic.cpp:286-339); F and r are (3,3,N) host arrays.
+ * Exposed for parity tests of a single evaluation. */
+int sc_rve_form_residual(sc_rve* rve, const double* F, double* r);
+
+/* Field / state access (host copies, (3,3,N) column-major). */
+enum {
+  SC_FIELD_F = 0,        /* solver deformation gradient (F_PETSc) */
+  SC_FIELD_P = 1,        /* first Piola-Kirchhoff stress of the last evaluation */
+  SC_FIELD_FP = 2,       /* plastic deformation gradient */
+  SC_FIELD_LP = 3,       /* plastic velocity gradient */
+  SC_FIELD_S = 4,        /* second Piola-Kirchhoff stress (lattice frame) */
+  SC_FIELD_FE = 5,       /* elastic deformation gradient */
+  SC_FIELD_F_LAST_INC = 6,
+  SC_FIELD_XI = 7,       /* (12,N) slip resistance */
+  SC_FIELD_GAMMA = 8,    /* (12,N) accumulated shear */
+  SC_FIELD_X_P = 9,      /* (3,N) cell centre coordinates (after update_coords) */
+  SC_FIELD_X_N = 10      /* (3,(Nx+1)(Ny+1)(Nz+1)) node coordinates */
+};
+int sc_rve_get_field(sc_rve* rve, int field, double* out, int64_t n_values);
+int sc_rve_set_field(sc_rve* rve, int field, const double* in, int64_t n_values);
+
+/* Solver scalars: P_av(9) P_aim(9) F_aim(9) F_aim_dot(9) F_aim_last_inc(9)
+ * C_volAvg(81) C_minMaxAvg(81) S(81) C_ref(81), err_div, err_BC. */
+typedef struct sc_solver_scalars {
+  double P_av[9], P_aim[9], F_aim[9], F_aim_dot[9], F_aim_last_inc[9];
+  double C_volAvg[81], C_minMaxAvg[81], S[81], C_ref[81];
+  double err_div, err_BC;
+  int    total_iter;
+} sc_solver_scalars;
+int sc_rve_get_scalars(sc_rve* rve, sc_solver_scalars* out);
+int sc_rve_get_stats(sc_rve* rve, sc_stats* out);
+int sc_rve_reset_stats(sc_rve* rve);
+
+/* ------------------------------------------------------------------------- */
+/* Spectral building blocks (parity tests against the reference's goldens).  */
+/* All arrays are host arrays; the product runs them on the device.          */
+/* ------------------------------------------------------------------------- */
+
+/* Spectral::init xi tables (spectral.cpp:53-91): xi1st/xi2nd as interleaved
+ * complex (re,im) of shape (3, Nx/2+1, Nz, Ny). */
+int sc_spectral_xi(const int cells[3], const double size[3], int derivative,
+                   double* xi1st, double* xi2nd);
+
+/* 3D real-to-complex forward FFT (unnormalised, e^{-i}) of `howmany`
+ * interleaved components: in (howmany, Nx, Ny, Nz) real, out (howmany,
+ * Nx/2+1, Nz, Ny) complex interleaved (the FFTW_MPI_TRANSPOSED_OUT layout of
+ * fft.cpp:48-52). */
+int sc_fft_r2c(const int cells[3], int howmany, const double* in, double* out);
+/* Inverse (e^{+i}), scaled by wgt as FFT::backward (fft.h:50-60). */
+int sc_fft_c2r(const int cells[3], int howmany, const double* in, double wgt, double* out);
+
+/* MechBase::calculate_divergence_rms, mech_base.cpp:302-398. field (3,3,N). */
+int sc_divergence_rms(const int cells[3], const double size[3], const double scaled_size[3],
+                      const double* xi1st, const double* field, double* rms);
+
+/* MechBase::gamma_convolution with the operator as intended
+ * (mech_base.cpp:400-477, Gamma_lmno = (A^-1)_ln conj(-xi_o) xi_m), C_ref =
+ * C / wgt, DC = field_aim / wgt, normalised inverse.  field (3,3,N) in place. */
+int sc_gamma_convolution(const int cells[3], const double* xi1st, const double* C_ref,
+                         const double field_aim[9], double* field);
+
+/* MechBase::calculate_masked_compliance, mech_base.cpp:249-300. */
+int sc_masked_compliance(const double C[81], const double rot_q[4], const int stress_mask[9],
+                         double S_out[81]);
+
+/* MechBase::base_init scaled geometry, mech_base.cpp:11-33. */
+int sc_scaled_geom_size(const int cells[3], const double size[3], int divergence_correction,
+                        double scaled[3]);
+
+/* MechBase::forward_field (mech_base.cpp:213-247) and calculate_rate
+ * (:479-497) on (3,3,N) fields. aim may be NULL. */
+int sc_forward_field(int64_t n_cells, double dt, const double* field_last_inc,
+                     const double* rate, const double* aim, double* out);
+int sc_calculate_rate(int64_t n_cells, int heterogeneous, const double* field0,
+                      const double* field, double dt, const double avRate[9], double* out);
+
+/* P_av / C_volAvg / C_minMaxAvg reductions of Spectral::constitutive_response
+ * (spectral.cpp:160-217) on given per-point P (3,3,N) and dPdF (3,3,3,3,N). */
+int sc_response_reductions(int64_t n_cells, const double* P, const double* dPdF,
+                           const double* rot_q /* may be NULL */,
+                           double P_av[9], double C_volAvg[81], double C_minMaxAvg[81]);
+
+/* ------------------------------------------------------------------------- */
+/* Material-point building blocks                                             */
+/* ------------------------------------------------------------------------- */
+
+/* Batched material-point update for n independent points (the inner loop of
+ * homogenization_mechanical_response, homogenization.f90:219-266, followed by
+ * mechanical_homogenize, homogenization_mechanical.f90:128-151).
+ *
+ * Inputs per point (column-major 3x3, SoA over points is NOT used here:
+ * arrays are (3,3,n) / (12,n) as in the reference):
+ *   F, F0 (phase F0), Fp0, Lp0, S0, xi0, gamma0 -- converged values of the last increment
+ *   Lp  -- in: Newton initial guess (Lp of the previous evaluation); out: Lp
+ * Outputs: P, Fp, S, Fe, xi, gamma, dPdF (may be NULL), broken (n, may be NULL),
+ *          iterations (2n: Newton total, FPI total; may be NULL).
+ * Orientation quaternions are not needed: Fp carries the orientation. */
+int sc_mp_update(int64_t n, double dt, const sc_material* m, const sc_phase_numerics* num,
+                 const double* F, const double* F0, const double* Fp0, const double* Lp0,
+                 const double* S0, const double* xi0, const double* gamma0,
+                 double* Lp, double* P, double* Fp, double* S, double* Fe,
+                 double* xi, double* gamma, double* dPdF, int* broken, int* iterations);
+
+/* Orientation matrix of a unit quaternion, qu2om with P = -1
+ * (rotations.f90:423-447); out (3,3) column-major. */
+void sc_qu2om(const double q[4], double om[9]);
+/* math_rotationalPart (math.f90:1117-1156), used by crystallite_orientations. */
+void sc_rotational_part(const double F[9], double R[9]);
+
+/* ------------------------------------------------------------------------- */
+/* Drop-in boundary: the Fortran bind(C) exports the reference's C++ grid     */
+/* solver calls (src/grid/spectral.h:16-23, damask_grid.h:16-20,              */
+/* main.cpp:8-14, discretization_grid.h:9-22, utilities_fortran.h:8-15).      */
+/* Arrays behind the fetched pointers are host mirrors owned by the library;  */
+/* in the product the material-point state stays resident on the device.     */
+/* ------------------------------------------------------------------------- */
+
+/* materialpoint.f90:68-117.  material/load/grid/numerics are paths
+ * (material.yaml, load.yaml, geometry .vti, numerics.yaml); the lengths are
+ * passed by value as in the reference. */
+void f_materialpoint_initBase(const char* material, int n1, const char* load, int n2,
+                              const char* grid, int n3, const char* numerics, int n4);
+/* materialpoint.f90:138-156 (phase, homogenization and state allocation). */
+void f_materialpoint_initDamask(void);
+/* materialpoint.f90:208-215: state0 <- state, Fp0 <- Fp, ... */
+void f_materialpoint_forward(void);
+/* materialpoint.f90:221-234 (result output; here: records the increment). */
+void f_materialpoint_result(int* inc, double* t);
+
+/* homogenization.f90:491-510 */
+void f_homogenization_fetch_tensor_pointers(double** F0, double** F, double** P,
+                                            double** dPdF, void** terminally_ill);
+/* homogenization.f90:219-266, 1-based inclusive cell range. */
+void f_homogenization_mechanical_response(double* dt, int* cell_start, int* cell_end);
+/* homogenization.f90:272 (no thermal field is active: no-op). */
+void f_homogenization_thermal_response(double* dt, int* cell_start, int* cell_end);
+/* homogenization.f90:299-324 (orientations + P, dPdF homogenization). */
+void f_homogenization_mechanical_response2(double* dt, int* execIP, int* execElem);
+
+/* discretization.f90:79-100, :145 */
+void f_discretization_init(int* material_at, int* n_cells, double* ip0, int* n_ip,
+                           double* node0, int* n_node, int* shared_nodes_begin);
+void f_discretization_fetch_ip_node_coord_pointers(double** ip, double** node);
+
+/* math.f90:599 (n x n inverse, err = 1 on failure), :819, :849, :428 */
+void f_math_invert(double* invA, int* err, double* A, int* n);
+void f_math_3333to99(double* m3333, double* m99);
+void f_math_99to3333(double* m99, double* m3333);
+void f_math_mul3333xx33(double* A, double* B, double* res);
+/* rotations.f90:331-360: passive unless *active == 1. */
+void f_rotate_tensor2(double* q, double* T, double* out, int* active);
+void f_rotate_tensor4(double* q, double* T, double* out, int* active);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SOLVER_CPP_AMD_H */

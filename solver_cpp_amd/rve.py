@@ -1,0 +1,169 @@
+"""Host-side mirror of the reference's grid solver interface.
+
+``Rve`` wraps one ``sc_rve`` context of a loaded library and exposes the
+entry points of ``MechSolverBasic`` / ``DamaskGrid`` (src/grid/mech_solver_basic.h,
+src/grid/damask_grid.h): ``init``, ``forward``, ``calculate_solution``,
+``update_coords`` and ``loop_over_loadcase``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .capi import (SC_DEF, SC_FIELD, SC_STRESS, GridNumerics, Library, LoadStep, Material, PhaseNumerics,
+                   RveDesc, Solution, SolverScalars, Stats, ptr)
+
+
+@dataclass
+class Load:
+    """One load step (Config::LoadStep, src/grid/config.h:86-96).
+
+    ``deformation`` / ``stress`` are 3x3 nested lists whose entries are
+    numbers or the string ``'x'`` (not prescribed), as in the load YAML.
+    """
+    deformation: list
+    stress: list
+    deformation_type: str = "dot_F"
+    stress_type: str = "P"
+    t: float = 5.0
+    N: int = 10
+    r: float = 1.0
+    rot_q: Sequence[float] = (1.0, 0.0, 0.0, 0.0)
+    estimate_rate: bool = True
+    f_out: int = 1
+
+    def to_c(self) -> LoadStep:
+        s = LoadStep()
+        s.deformation_type = SC_DEF[self.deformation_type]
+        s.stress_type = SC_STRESS[self.stress_type]
+        for i in range(3):
+            for j in range(3):
+                q = i + 3 * j  # column-major
+                d = self.deformation[i][j]
+                p = self.stress[i][j]
+                s.deformation_mask[q] = 1 if d == "x" else 0
+                s.deformation[q] = 0.0 if d == "x" else float(d)
+                s.stress_mask[q] = 1 if p == "x" else 0
+                s.stress[q] = 0.0 if p == "x" else float(p)
+        for k in range(4):
+            s.rot_q[k] = float(self.rot_q[k])
+        s.t, s.N, s.r = float(self.t), int(self.N), float(self.r)
+        s.estimate_rate = 1 if self.estimate_rate else 0
+        s.f_out = int(self.f_out)
+        return s
+
+
+def tension_x(t: float = 5.0, N: int = 10, rate: float = 1.0e-3, rot_q=(1.0, 0.0, 0.0, 0.0)) -> Load:
+    """examples/grid/tensionX.yaml: dot_F_xx = rate, P_yy = P_zz = 0."""
+    return Load(deformation=[[rate, 0, 0], [0, "x", 0], [0, 0, "x"]],
+                stress=[["x", "x", "x"], ["x", 0, "x"], ["x", "x", 0]], t=t, N=N, rot_q=rot_q)
+
+
+class Rve:
+    """One representative volume element on one library backend."""
+
+    def __init__(self, lib: Library, cells, size, material_at: np.ndarray, orientations: np.ndarray,
+                 material: Optional[Material] = None, phase_num: Optional[PhaseNumerics] = None,
+                 grid_num: Optional[GridNumerics] = None, precision: int = 0, **grid_overrides):
+        self.lib = lib
+        m, p, g = lib.defaults()
+        if material is not None:
+            m = material
+        if phase_num is not None:
+            p = phase_num
+        if grid_num is not None:
+            g = grid_num
+        for k, v in grid_overrides.items():
+            setattr(g, k, v)
+        self.cells = tuple(int(c) for c in cells)
+        self.N = int(np.prod(self.cells))
+        self._mat = np.ascontiguousarray(material_at, dtype=np.int32).reshape(-1)
+        self._ori = np.ascontiguousarray(orientations, dtype=np.float64).reshape(-1, 4)
+        assert self._mat.size == self.N
+        d = RveDesc()
+        for k in range(3):
+            d.cells[k] = self.cells[k]
+            d.size[k] = float(size[k])
+        d.n_materials = self._ori.shape[0]
+        d.orientations = ptr(self._ori)
+        d.material_at = ptr(self._mat, C.POINTER(C.c_int))
+        d.material, d.phase_num, d.grid_num = m, p, g
+        d.precision = int(precision)
+        self.desc = d
+        h = C.c_void_p()
+        lib.check(lib.sc_rve_create(C.byref(d), C.byref(h)), "sc_rve_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.sc_rve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # MechSolverBasic entry points --------------------------------------------
+    def init(self):
+        self.lib.check(self.lib.sc_rve_init(self.h), "sc_rve_init")
+
+    def forward(self, cut_back: bool, guess: bool, dt: float, dt_old: float, t_remaining: float, load: Load):
+        st = load.to_c()
+        self.lib.check(self.lib.sc_rve_forward(self.h, int(cut_back), int(guess), dt, dt_old, t_remaining,
+                                               C.byref(st)), "sc_rve_forward")
+
+    def calculate_solution(self) -> Solution:
+        s = Solution()
+        self.lib.check(self.lib.sc_rve_solution(self.h, C.byref(s)), "sc_rve_solution")
+        return s
+
+    def update_coords(self):
+        self.lib.check(self.lib.sc_rve_update_coords(self.h), "sc_rve_update_coords")
+
+    def loop_over_loadcase(self, loads: Sequence[Load]):
+        """DamaskGrid::loop_over_loadcase; returns per-increment (iterations, evaluations)."""
+        steps = (LoadStep * len(loads))(*[l.to_c() for l in loads])
+        n = sum(int(l.N) for l in loads)
+        its = np.zeros(n, dtype=np.int32)
+        evs = np.zeros(n, dtype=np.int32)
+        rc = self.lib.sc_rve_run_loadcase(self.h, steps, len(loads), ptr(its, C.POINTER(C.c_int)),
+                                          ptr(evs, C.POINTER(C.c_int)), n)
+        self.lib.check(rc, "sc_rve_run_loadcase")
+        return its, evs
+
+    def form_residual(self, F: np.ndarray) -> np.ndarray:
+        F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+        r = np.zeros_like(F)
+        self.lib.check(self.lib.sc_rve_form_residual(self.h, ptr(F), ptr(r)), "sc_rve_form_residual")
+        return r
+
+    # access -------------------------------------------------------------------
+    def field(self, name: str) -> np.ndarray:
+        n = {"xi": 12, "gamma": 12, "x_p": 3}.get(name, 9) * self.N
+        if name == "x_n":
+            n = 3 * (self.cells[0] + 1) * (self.cells[1] + 1) * (self.cells[2] + 1)
+        out = np.zeros(n)
+        self.lib.check(self.lib.sc_rve_get_field(self.h, SC_FIELD[name], ptr(out), n), f"get_field {name}")
+        return out
+
+    def set_field(self, name: str, values: np.ndarray):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        self.lib.check(self.lib.sc_rve_set_field(self.h, SC_FIELD[name], ptr(v), v.size), f"set_field {name}")
+
+    def scalars(self) -> SolverScalars:
+        s = SolverScalars()
+        self.lib.check(self.lib.sc_rve_get_scalars(self.h, C.byref(s)), "get_scalars")
+        return s
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self.lib.check(self.lib.sc_rve_get_stats(self.h, C.byref(s)), "get_stats")
+        return s
+
+    def reset_stats(self):
+        self.lib.check(self.lib.sc_rve_reset_stats(self.h), "reset_stats")
