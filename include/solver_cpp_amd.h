@@ -177,6 +177,14 @@ int sc_rve_run_loadcase(sc_rve* rve, const sc_loadstep* steps, int n_steps,
  * Exposed for parity tests of a single evaluation. */
 int sc_rve_form_residual(sc_rve* rve, const double* F, double* r);
 
+/* Replay of the outer nonlinear solver (PETSc SNESNGMRES with the options of
+ * mech_solver_basic.cpp:29-31, m = history size): starting from x0 (n
+ * values), the k-th function evaluation returns r_rec[k*n .. k*n+n) and the
+ * point it was requested at is written to x_eval[k*n ..).  The solve is
+ * declared converged once n_evals evaluations were made.  Used to check the
+ * solver arithmetic against a recorded trace. */
+int sc_ngmres_replay(int64_t n, int m, const double* x0, int n_evals, const double* r_rec, double* x_eval);
+
 /* Field / state access (host copies, (3,3,N) column-major). */
 enum {
   SC_FIELD_F = 0,        /* solver deformation gradient (F_PETSc) */
@@ -229,8 +237,10 @@ int sc_divergence_rms(const int cells[3], const double size[3], const double sca
                       const double* xi1st, const double* field, double* rms);
 
 /* MechBase::gamma_convolution with the operator as intended
- * (mech_base.cpp:400-477, Gamma_lmno = (A^-1)_ln conj(-xi_o) xi_m), C_ref =
- * C / wgt, DC = field_aim / wgt, normalised inverse.  field (3,3,N) in place. */
+ * (mech_base.cpp:400-477, Gamma_lmno = (A^-1)_ln conj(-xi_o) xi_m, from the
+ * commented-out fill at :188-205).  C_ref is the reference stiffness as
+ * stored by update_gamma (C / wgt); DC = field_aim; unnormalised inverse.
+ * field (3,3,N) in place. */
 int sc_gamma_convolution(const int cells[3], const double* xi1st, const double* C_ref,
                          const double field_aim[9], double* field);
 

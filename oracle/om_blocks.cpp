@@ -73,7 +73,7 @@ int sc_gamma_convolution(const int cells[3], const double* xi1st, const double* 
   std::vector<cplx> hat;
   tensor_r2c(g, field, hat);
   gamma_apply(g, load3333(C_ref), hat, load33(field_aim));
-  tensor_c2r(g, hat, g.wgt, field);
+  tensor_c2r(g, hat, 1.0, field);
   return 0;
 }
 

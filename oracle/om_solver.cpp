@@ -15,6 +15,7 @@
 #include <omp.h>
 
 #include <chrono>
+#include <functional>
 #include <cstring>
 #include <stdexcept>
 
@@ -155,7 +156,7 @@ void gamma_convolution(sc_rve* R, double* field, const M3& aim) {
   std::vector<cplx> hat;
   tensor_r2c(R->g, field, hat);
   gamma_apply(R->g, R->C_ref, hat, aim);
-  tensor_c2r(R->g, hat, R->g.wgt, field);
+  tensor_c2r(R->g, hat, 1.0, field);
 }
 
 double divergence(sc_rve* R, const double* field) {
@@ -271,26 +272,25 @@ void sym_lstsq(int l, std::vector<double> H, std::vector<double>& b) {
   b = x;
 }
 
-// SNESSolve with SNESNGMRES on F_PETSc.  Returns reason (>0 converged).
-int snes_ngmres(sc_rve* R) {
-  const int64_t n = 9 * R->g.N;
-  const int msize = R->desc.grid_num.ngmres_m > 0 ? R->desc.grid_num.ngmres_m : 30;
+// SNESSolve with SNESNGMRES.  `residual(x, f)` evaluates the function,
+// `conv(k)` is the convergence test after iteration k (1 converged, -1
+// diverged, 0 iterating); `set_iter(k)` mirrors snes->iter.  Returns the
+// SNES converged reason (> 0 converged).
+int ngmres_solve(int64_t n, int msize, std::vector<double>& X,
+                 const std::function<void(const std::vector<double>&, std::vector<double>&)>& residual,
+                 const std::function<int(int)>& conv, const std::function<void(int)>& set_iter) {
   const int max_its = 50;
   const double gammaA = 2.0, gammaC = 2.0, epsilonB = 0.1, deltaB = 0.9;
   const int restart_it = 2;
   std::vector<std::vector<double>> Fdot(msize), Xdot(msize);
   std::vector<double> fnorms(msize, 0.0), Q(msize * msize, 0.0);
-  std::vector<double>& X = R->F;
   std::vector<double> Fv(n), XM(n), FM(n), XA(n), FA(n);
-  R->snes_iter = 0;
-  R->nfuncs = 0;
-  R->solve_evals = 0;
-
-  form_residual(R, X.data(), Fv.data());
+  set_iter(0);
+  residual(X, Fv);
   double fnorm = vnorm(Fv);
   if (std::isnan(fnorm) || std::isinf(fnorm)) return -4;  // SNES_DIVERGED_FNORM_NAN
   double fminnorm = fnorm;
-  int reason = converged(R);
+  int reason = conv(0);
   if (reason) return reason > 0 ? 6 : -5;
   auto update_subspace = [&](int ivec, const std::vector<double>& f, double fn, const std::vector<double>& x) {
     Fdot[ivec] = f;
@@ -300,11 +300,11 @@ int snes_ngmres(sc_rve* R) {
   update_subspace(0, Fv, fnorm, X);
   int k_restart = 1, l = 1, ivec = 0, restart_count = 0;
   for (int k = 1; k < max_its + 1; ++k) {
-    // x^M: basic line search, XM = X - F
+    // x^M: basic line search with damping 1, XM = X - F
     for (int64_t i = 0; i < n; ++i) XM[i] = X[i] - 1.0 * Fv[i];
-    form_residual(R, XM.data(), FM.data());
+    residual(XM, FM);
     double fMnorm = vnorm(FM);
-    if (std::isnan(fMnorm) || std::isinf(fMnorm)) return -6;  // line search failure
+    if (std::isnan(fMnorm) || std::isinf(fMnorm)) return -6;  // SNES_DIVERGED_LINE_SEARCH
     // SNESNGMRESFormCombinedSolution_Private
     {
       double nu = fMnorm * fMnorm;
@@ -331,7 +331,7 @@ int snes_ngmres(sc_rve* R) {
         for (int i = 0; i < l; ++i) v += beta[i] * Xdot[i][t];
         XA[t] = v;
       }
-      form_residual(R, XA.data(), FA.data());
+      residual(XA, FA);
     }
     if (fminnorm > fMnorm) fminnorm = fMnorm;
     double dnorm = vdiffnorm(XA, XM);
@@ -342,7 +342,7 @@ int snes_ngmres(sc_rve* R) {
     }
     double fAnorm = vnorm(FA);
     if (std::isnan(fnorm) || std::isinf(fnorm)) return -4;
-    // SNESNGMRESSelect_Private (difference)
+    // SNESNGMRESSelect_Private, difference selection
     bool selectA = fAnorm < gammaA * fminnorm;
     if (!(epsilonB * dnorm < dminnorm || std::sqrt(fnorm) < deltaB * std::sqrt(fminnorm))) selectA = false;
     if (selectA) {
@@ -354,7 +354,7 @@ int snes_ngmres(sc_rve* R) {
       Fv = FM;
       X = XM;
     }
-    // SNESNGMRESSelectRestart_Private (difference)
+    // SNESNGMRESSelectRestart_Private, difference restart
     bool selectRestart = false;
     if (epsilonB * dnorm > dminnorm && fAnorm > deltaB * fminnorm && l > 0) selectRestart = true;
     if (fAnorm > gammaC * fminnorm) selectRestart = true;
@@ -372,11 +372,22 @@ int snes_ngmres(sc_rve* R) {
       if (fminnorm > fnorm) fminnorm = fnorm;
       update_subspace(ivec, Fv, fnorm, X);
     }
-    R->snes_iter = k;
-    reason = converged(R);
+    set_iter(k);
+    reason = conv(k);
     if (reason) return reason > 0 ? 6 : -5;
   }
   return -5;  // SNES_DIVERGED_MAX_IT
+}
+
+int snes_ngmres(sc_rve* R) {
+  const int msize = R->desc.grid_num.ngmres_m > 0 ? R->desc.grid_num.ngmres_m : 30;
+  R->snes_iter = 0;
+  R->nfuncs = 0;
+  R->solve_evals = 0;
+  return ngmres_solve(
+      9 * R->g.N, msize, R->F,
+      [R](const std::vector<double>& x, std::vector<double>& f) { form_residual(R, x.data(), f.data()); },
+      [R](int) { return converged(R); }, [R](int k) { R->snes_iter = k; });
 }
 
 void base_update_coords(sc_rve* R) {
@@ -729,6 +740,26 @@ int sc_rve_run_loadcase(sc_rve* R, const sc_loadstep* steps, int n_steps, int* i
     return 0;
   } catch (std::exception& e) {
     g_err = e.what();
+    return 1;
+  }
+}
+
+int sc_ngmres_replay(int64_t n, int m, const double* x0, int n_evals, const double* r_rec, double* x_eval) {
+  try {
+    std::vector<double> X(x0, x0 + n);
+    int e = 0;
+    ngmres_solve(
+        n, m, X,
+        [&](const std::vector<double>& x, std::vector<double>& f) {
+          if (e >= n_evals) throw std::runtime_error("replay ran past the recorded evaluations");
+          std::memcpy(x_eval + (int64_t)e * n, x.data(), n * sizeof(double));
+          std::memcpy(f.data(), r_rec + (int64_t)e * n, n * sizeof(double));
+          e++;
+        },
+        [&](int) { return e >= n_evals ? 1 : 0; }, [](int) {});
+    return e == n_evals ? 0 : 3;
+  } catch (std::exception& ex) {
+    g_err = ex.what();
     return 1;
   }
 }

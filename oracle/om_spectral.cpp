@@ -141,9 +141,11 @@ double divergence_rms(const Grid& g, const std::vector<cplx>& Phat) {
 }
 
 // gamma_convolution body (memory-efficient branch) with the intended
-// operator Gamma_lmno = (A^-1)_ln conj(-xi_o) xi_m when |det Re A| > 1e-16,
-// DC = aim / wgt (DAMASK's normalised convention; the caller applies the
-// inverse transform scaled by wgt).
+// operator Gamma_lmno = (A^-1)_ln conj(-xi_o) xi_m when |det Re A| > 1e-16
+// (mech_base.cpp:188-205).  With C_ref = C / wgt (mech_base.cpp:157), an
+// unnormalised forward transform and an unnormalised inverse (neutral_wgt = 1,
+// mech_base.cpp:474-476) the non-DC modes give Gamma(C) * P exactly, and the
+// DC mode is set to the aim itself (mech_base.cpp:467-473).
 void gamma_apply(const Grid& g, const T4& C_ref, std::vector<cplx>& Phat, const M3& aim) {
   const int Ny = g.cells[1], Nz = g.cells[2], nr = g.nr;
   for (int j = 0; j < Ny; ++j)
@@ -196,7 +198,7 @@ void gamma_apply(const Grid& g, const T4& C_ref, std::vector<cplx>& Phat, const 
           for (int m = 0; m < 3; ++m) Phat[9 * kk + l + 3 * m] = res[l][m];
       }
   for (int l = 0; l < 3; ++l)
-    for (int m = 0; m < 3; ++m) Phat[l + 3 * m] = cplx(aim.a[l][m] / g.wgt, 0.0);
+    for (int m = 0; m < 3; ++m) Phat[l + 3 * m] = cplx(aim.a[l][m], 0.0);
 }
 
 // calculate_masked_compliance, mech_base.cpp:249-300.  The stress mask is

@@ -110,6 +110,7 @@ _SIGNATURES = {
     "sc_rve_update_coords": (C.c_int, [C.c_void_p]),
     "sc_rve_run_loadcase": (C.c_int, [C.c_void_p, C.POINTER(LoadStep), C.c_int, _ip, _ip, C.c_int]),
     "sc_rve_form_residual": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "sc_ngmres_replay": (C.c_int, [C.c_int64, C.c_int, _dp, C.c_int, _dp, _dp]),
     "sc_rve_get_field": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int64]),
     "sc_rve_set_field": (C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int64]),
     "sc_rve_get_scalars": (C.c_int, [C.c_void_p, C.POINTER(SolverScalars)]),
@@ -126,7 +127,7 @@ _SIGNATURES = {
     "sc_calculate_rate": (C.c_int, [C.c_int64, C.c_int, _dp, _dp, C.c_double, _dp, _dp]),
     "sc_response_reductions": (C.c_int, [C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]),
     "sc_mp_update": (C.c_int, [C.c_int64, C.c_double, C.POINTER(Material), C.POINTER(PhaseNumerics)]
-                     + [_dp] * 7 + [_dp] * 8 + [_dp, _ip, _ip]),
+                     + [_dp] * 7 + [_dp] * 7 + [_dp, _ip, _ip]),
     "sc_qu2om": (None, [_dp, _dp]),
     "sc_rotational_part": (None, [_dp, _dp]),
     "f_math_invert": (None, [_dp, _ip, _dp, _ip]),

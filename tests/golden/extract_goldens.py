@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Write the reference's golden vectors as small JSON/NPZ fixtures.
+
+Reads (as text) the reference's GoogleTest sources under
+``src/grid/test/`` and the original-DAMASK stdout trace
+``output_damask_f.sh``; writes ``tests/golden/reference_goldens.json`` and
+``tests/golden/fortran_trace.npz``.  Only data (inputs and expected outputs)
+is written; no reference source text is kept.
+
+Usage: python tests/golden/extract_goldens.py [/root/reference]
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+TESTDIR = os.path.join(REF, "src", "grid", "test")
+
+_NUM = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?"
+
+
+def _parse_braces(text: str, start: int):
+    """Parse a nested brace list starting at text[start] == '{'.
+    Returns (nested python list, index after the closing brace)."""
+    assert text[start] == "{"
+    stack = [[]]
+    i = start + 1
+    while stack:
+        ch = text[i]
+        if ch == "{":
+            stack.append([])
+            i += 1
+        elif ch == "}":
+            top = stack.pop()
+            if stack:
+                stack[-1].append(top)
+            else:
+                return top, i + 1
+            i += 1
+        elif text.startswith("c(", i):
+            j = text.index(")", i)
+            re_, im_ = [float(x) for x in text[i + 2:j].split(",")]
+            stack[-1].append([re_, im_])
+            i = j + 1
+        else:
+            m = re.match(_NUM, text[i:])
+            if m and (i == 0 or not (text[i - 1].isalnum() or text[i - 1] == "_")):
+                stack[-1].append(float(m.group(0)))
+                i += len(m.group(0))
+            else:
+                i += 1
+    raise ValueError("unbalanced")
+
+
+def set_values(src: str, name: str, occurrence: int = 0):
+    """Nested list of `name.setValues({...})` (first index outermost)."""
+    pat = re.compile(re.escape(name) + r"\.setValues\(\{")
+    matches = list(pat.finditer(src))
+    m = matches[occurrence]
+    lst, _ = _parse_braces(src, m.end() - 1)
+    return lst
+
+
+def in_test(src: str, test: str) -> str:
+    i = src.index(test)
+    j = src.find("TEST_F(", i + 1)
+    return src[i: j if j > 0 else len(src)]
+
+
+def tensor_colmajor(nested, shape, complex_=False):
+    """Nested list with first index outermost -> flat Fortran-order list."""
+    a = np.array(nested, dtype=np.float64)
+    if complex_:
+        a = a[..., 0] + 1j * a[..., 1]
+    a = a.reshape(shape)
+    flat = a.reshape(-1, order="F")
+    if complex_:
+        return {"re": flat.real.tolist(), "im": flat.imag.tolist(), "shape": list(shape)}
+    return {"data": flat.tolist(), "shape": list(shape)}
+
+
+def read(name):
+    with open(os.path.join(TESTDIR, name)) as f:
+        return f.read()
+
+
+def main():
+    out = {}
+    sp = read("test_spectral.cpp")
+    t = in_test(sp, "SpectralTestInit")
+    out["spectral_init_2x1x1"] = {
+        "cells": [2, 1, 1], "size": [2e-5, 1e-5, 1e-5], "derivative": 0,
+        "xi1st": tensor_colmajor(set_values(t, "expected_xi1st"), (3, 2, 1, 1), True),
+        "xi2nd": tensor_colmajor(set_values(t, "expected_xi2nd"), (3, 2, 1, 1), True),
+        "source": "src/grid/test/test_spectral.cpp:15-48"}
+    t = in_test(sp, "SpectralTestConstitutiveResponse")
+    out["constitutive_response_2x1x1"] = {
+        "F": tensor_colmajor(set_values(t, "F"), (3, 3, 2, 1, 1)),
+        "P": tensor_colmajor(set_values(t, "mech2_homogenization_P_result"), (3, 3, 2)),
+        "dPdF": tensor_colmajor(set_values(t, "mech2_homogenization_dPdF_result"), (3, 3, 3, 3, 2)),
+        "P_av": tensor_colmajor(set_values(t, "expected_P_av"), (3, 3)),
+        "C_volAvg": tensor_colmajor(set_values(t, "expected_C_vol_avg"), (3, 3, 3, 3)),
+        "C_minMaxAvg": tensor_colmajor(set_values(t, "expected_C_minmax_avg"), (3, 3, 3, 3)),
+        "source": "src/grid/test/test_spectral.cpp:58-216"}
+
+    mb = read("test_mech_base.cpp")
+    t = in_test(mb, "MechBaseTestInitDivergenceCorrection")
+    out["scaled_geom_size"] = {"cells": [2, 1, 1], "size": [3, 4, 5], "none": [3, 4, 5],
+                               "size_only": [0.75, 1, 1.25], "size_grid": [0.6, 0.8, 1],
+                               "source": "src/grid/test/test_mech_base.cpp:55-77"}
+    t = in_test(mb, "MechBaseTestUpdateCoords")
+    out["update_coords_2x1x1"] = {
+        "F": tensor_colmajor(set_values(t, "F_"), (3, 3, 2, 1, 1)),
+        "x_n": tensor_colmajor(set_values(t, "expected_x_n"), (3, 3, 2, 2)),
+        "x_p": tensor_colmajor(set_values(t, "expected_x_p"), (3, 2, 1, 1)),
+        "source": "src/grid/test/test_mech_base.cpp:79-152"}
+    t = in_test(mb, "MechBaseTestForwardField")
+    out["forward_field_2x1x1"] = {
+        "field_last_inc": tensor_colmajor(set_values(t, "field_last_inc"), (3, 3, 2, 1, 1)),
+        "rate": tensor_colmajor(set_values(t, "rate"), (3, 3, 2, 1, 1)),
+        "aim": tensor_colmajor(set_values(t, "aim"), (3, 3)),
+        "delta_t": 1.0,
+        "expected": tensor_colmajor(set_values(t, "expected_forwarded_field"), (3, 3, 2, 1, 1)),
+        "source": "src/grid/test/test_mech_base.cpp:199-237"}
+    t = in_test(mb, "MechBaseTestMaskedCompliance")
+    out["masked_compliance"] = {
+        "C": tensor_colmajor(set_values(t, "C"), (3, 3, 3, 3)),
+        "rot_q": [1.0, 0.0, 0.0, 0.0],
+        "stress_mask_rowmajor": [[1, 1, 1], [1, 0, 1], [1, 1, 0]],
+        "expected": tensor_colmajor(set_values(t, "expected_masked_compliance"), (3, 3, 3, 3)),
+        "source": "src/grid/test/test_mech_base.cpp:239-321"}
+    t = in_test(mb, "MechBaseTestDivergenceRMS")
+    rms = float(re.search(r"ASSERT_EQ\(rms,\s*(" + _NUM + r")\)", t).group(1))
+    out["divergence_rms_4x1x1"] = {
+        "cells": [4, 1, 1], "size": [4e-5, 1e-5, 1e-5], "scaled_size": [4, 1, 1], "wgt": 0.25,
+        "field": tensor_colmajor(set_values(t, "tensor_field"), (3, 3, 4, 1, 1)),
+        "xi1st": tensor_colmajor(set_values(t, "spectral.xi1st"), (3, 3, 1, 1), True),
+        "expected_rms": rms,
+        "source": "src/grid/test/test_mech_base.cpp:323-387"}
+    t = in_test(mb, "MechBaseTestGammaConvolution")
+    out["gamma_convolution_2x1x1"] = {
+        "field_aim": tensor_colmajor(set_values(t, "field_aim"), (3, 3)),
+        "C_ref": tensor_colmajor(set_values(t, "mech_base.C_ref"), (3, 3, 3, 3)),
+        "field": tensor_colmajor(set_values(t, "field"), (3, 3, 2, 1, 1)),
+        "expected": tensor_colmajor(set_values(t, "expected_gamma_field"), (3, 3, 2, 1, 1)),
+        "source": "src/grid/test/test_mech_base.cpp:389-490"}
+    t = in_test(mb, "TestCalculateRate")
+    out["calculate_rate_2x1x1"] = {
+        "field": tensor_colmajor(set_values(t, "field"), (3, 3, 2, 1, 1)),
+        "field_0": tensor_colmajor(set_values(t, "field_0"), (3, 3, 2, 1, 1)),
+        "avRate": tensor_colmajor(set_values(t, "avRate"), (3, 3)),
+        "dt": 100.2, "heterogeneous": 1,
+        "expected": tensor_colmajor(set_values(t, "expected_rate"), (3, 3, 2, 1, 1)),
+        "source": "src/grid/test/test_mech_base.cpp:492-545"}
+
+    ms = read("test_mech_solver_basic.cpp")
+    t = in_test(ms, "TestBasicForward")
+    out["basic_forward_2x1x1"] = {
+        "dot_F": [[1e-3, 0, 0], [0, "x", 0], [0, 0, "x"]],
+        "P": [["x", "x", "x"], ["x", 0, "x"], ["x", "x", 0]],
+        "cut_back": 0, "guess": 0, "delta_t": 100.2, "delta_t_old": 1.0, "t_remaining": 1002.0,
+        "expected_aim": tensor_colmajor(set_values(t, "expected_aim"), (3, 3)),
+        "expected_F": tensor_colmajor(set_values(t, "forwarded_field"), (3, 3, 2, 1, 1)),
+        "source": "src/grid/test/test_mech_solver_basic.cpp:160-230"}
+    t = in_test(ms, "TestFormResidual")
+    err_bc = float(re.search(r"EXPECT_EQ\(mech_basic\.err_BC,\s*(" + _NUM + r")\)", t).group(1))
+    out["form_residual_bc"] = {
+        "S": tensor_colmajor(set_values(t, "mech_basic.S"), (3, 3, 3, 3)),
+        "P_av": tensor_colmajor(set_values(t, "mech_basic.P_av"), (3, 3)),
+        "F_aim": [[1.1002000000000001, 0, 0], [0, 1, 0], [0, 0, 1]],
+        "stress_mask_rowmajor": [[1, 1, 1], [1, 0, 1], [1, 1, 0]],
+        "expected_rotated": tensor_colmajor(set_values(t, "expected_rotated"), (3, 3)),
+        "expected_F_aim": tensor_colmajor(set_values(t, "expected_F_aim"), (3, 3)),
+        "expected_err_BC": err_bc,
+        "source": "src/grid/test/test_mech_solver_basic.cpp:232-376"}
+
+    with open(os.path.join(HERE, "reference_goldens.json"), "w") as f:
+        json.dump(out, f, indent=None, separators=(",", ":"))
+        f.write("\n")
+
+    # ---- original-DAMASK trace (3x3x3, 20 grains, tensionX t=5 N=10) ----------
+    with open(os.path.join(REF, "output_damask_f.sh"), errors="replace") as f:
+        lines = f.read().splitlines()
+    evals_F, evals_r, inc_of_eval, pav = [], [], [], []
+    inc = 0
+    conv_incs = []
+    cur_F = None
+    for ln in lines:
+        s = ln.strip()
+        m = re.match(r"Time\s+\S+s: Increment (\d+)/", s)
+        if m:
+            inc = int(m.group(1))
+            continue
+        if s.startswith("FR F1"):
+            cur_F = np.array([float(x) for x in s[5:].split()])
+            continue
+        if s.startswith("FR r2"):
+            r = np.array([float(x) for x in s[5:].split()])
+            assert cur_F is not None and cur_F.size == 243 and r.size == 243
+            evals_F.append(cur_F)
+            evals_r.append(r)
+            inc_of_eval.append(inc)
+            cur_F = None
+            continue
+        m = re.match(r"increment (\d+) converged", s)
+        if m:
+            conv_incs.append(int(m.group(1)))
+    np.savez_compressed(os.path.join(HERE, "fortran_trace.npz"), F=np.array(evals_F), r=np.array(evals_r),
+                        increment=np.array(inc_of_eval, dtype=np.int32),
+                        converged_increments=np.array(conv_incs, dtype=np.int32))
+    print(f"wrote {len(out)} golden groups, {len(evals_F)} trace evaluations")
+
+
+if __name__ == "__main__":
+    main()
