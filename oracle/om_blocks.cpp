@@ -122,23 +122,16 @@ int sc_response_reductions(int64_t n, const double* P, const double* dPdF, const
   M3 Pm = load33(pav);
   if (rot_q) Pm = rotTensor2(rot_q, Pm, false);
   store33(Pm, P_av);
-  double nmax = 0.0, nmin = std::numeric_limits<double>::max();
-  int64_t imax = -1, imin = -1;
-  std::vector<double> vol(81, 0.0);
+  std::vector<double> vol(81, 0.0), n2(n);
   for (int64_t c = 0; c < n; ++c) {
     const double* d = dPdF + 81 * c;
-    double n2 = 0.0;
-    for (int q = 0; q < 81; ++q) n2 += d[q] * d[q];
-    if (nmax < n2) {
-      nmax = n2;
-      imax = c;
-    }
-    if (nmin > n2) {
-      nmin = n2;
-      imin = c;
-    }
+    double s = 0.0;
+    for (int q = 0; q < 81; ++q) s += d[q] * d[q];
+    n2[c] = s;
     for (int q = 0; q < 81; ++q) vol[q] += d[q];
   }
+  int64_t imax = -1, imin = -1;
+  tangent_minmax_select(n2.data(), n, imax, imin);
   for (int q = 0; q < 81; ++q) {
     double a = imax >= 0 ? dPdF[81 * imax + q] : 0.0;
     double b = imin >= 0 ? dPdF[81 * imin + q] : std::numeric_limits<double>::max();

@@ -99,23 +99,17 @@ void constitutive_response(sc_rve* R, const double* F, double dt, const double* 
   R->P_av = load33(pav);
   if (rot_q) R->P_av = rotTensor2(rot_q, R->P_av, false);
   // C_minMaxAvg / C_volAvg
-  double nmax = 0.0, nmin = std::numeric_limits<double>::max();
-  int64_t imax = -1, imin = -1;
+  std::vector<double> n2(N);
   std::vector<double> vol(81, 0.0);
   for (int64_t c = 0; c < N; ++c) {
     const double* d = &R->homdPdF[81 * c];
-    double n2 = 0.0;
-    for (int q = 0; q < 81; ++q) n2 += d[q] * d[q];
-    if (nmax < n2) {
-      nmax = n2;
-      imax = c;
-    }
-    if (nmin > n2) {
-      nmin = n2;
-      imin = c;
-    }
+    double s = 0.0;
+    for (int q = 0; q < 81; ++q) s += d[q] * d[q];
+    n2[c] = s;
     for (int q = 0; q < 81; ++q) vol[q] += d[q];
   }
+  int64_t imax = -1, imin = -1;
+  tangent_minmax_select(n2.data(), N, imax, imin);
   double mm[81];
   for (int q = 0; q < 81; ++q) {
     double a = imax >= 0 ? R->homdPdF[81 * imax + q] : 0.0;

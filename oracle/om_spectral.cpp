@@ -241,3 +241,29 @@ T4 masked_compliance(const T4& C, const double q[4], const int mask[9]) {
 }
 
 }  // namespace orc
+
+namespace orc {
+// Points whose tangents form C_minMaxAvg (spectral.cpp:189-207 picks the
+// first cell of largest and of smallest squared norm |dPdF|^2 with exact
+// comparisons).  |dPdF|^2 is invariant under rotation, so grains that differ
+// only in orientation tie in exact arithmetic and the reference's choice
+// among them is decided by rounding noise of its own summation.  Here norms
+// within TANGENT_TIE_REL of the extreme count as tied and the first such
+// cell is taken: the same cells for any implementation whose norms agree to
+// far better than TANGENT_TIE_REL.
+void tangent_minmax_select(const double* n2, int64_t n, int64_t& imax, int64_t& imin) {
+  imax = imin = -1;
+  if (n <= 0) return;
+  double M = n2[0], m = n2[0];
+  for (int64_t c = 1; c < n; ++c) {
+    M = std::max(M, n2[c]);
+    m = std::min(m, n2[c]);
+  }
+  const double hi = M * (1.0 - TANGENT_TIE_REL), lo = m * (1.0 + TANGENT_TIE_REL);
+  for (int64_t c = 0; c < n && (imax < 0 || imin < 0); ++c) {
+    if (imax < 0 && n2[c] >= hi) imax = c;
+    if (imin < 0 && n2[c] <= lo) imin = c;
+  }
+}
+
+}  // namespace orc

@@ -143,4 +143,8 @@ double divergence_rms(const Grid& g, const std::vector<cplx>& Phat);
 void gamma_apply(const Grid& g, const T4& C_ref, std::vector<cplx>& Phat, const M3& aim);
 T4 masked_compliance(const T4& C, const double q[4], const int mask[9]);
 
+// selection of the C_minMaxAvg cells with a relative tie window
+constexpr double TANGENT_TIE_REL = 1e-10;
+void tangent_minmax_select(const double* n2, int64_t n, int64_t& imax, int64_t& imin);
+
 }  // namespace orc

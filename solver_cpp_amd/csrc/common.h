@@ -1,0 +1,82 @@
+// Shared definitions of the HIP solver library (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/solver_cpp_amd.h"
+
+namespace sca {
+
+#define SCA_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                               __FILE__ + ":" + std::to_string(__LINE__));                \
+  } while (0)
+
+constexpr int NSLIP = 12;
+constexpr int NSTATE = 24;
+
+// Material constants in the form the kernels use.  Slip system s has the
+// Schmid matrix P_s = v * sgn_s (sgn_s: entries in {-1,0,1}, compile-time in
+// the kernel), so everything built from P_s and the cubic stiffness is a
+// compile-time pattern times a few runtime scalars; hv are the 7 slip-slip
+// interaction values indexed by the compile-time cF interaction types.
+struct MatConst {
+  double C11, C12, C44;
+  double v;            // |P_s| entry magnitude, (1/sqrt 2)(1/sqrt 3)
+  double hv[7];
+  double xi_inf, h0, n, a, gdot0;
+  double atol_xi, atol_gamma;
+  // phase numerics
+  double r_cutback_min, r_cutback, r_increase, eps_rel_state;
+  int N_iter_state_max, N_iter_Lp_max;
+  double r_linesearch_Lp, eps_rel_Lp, eps_abs_Lp;
+  // exponents n_sl and a_sl as multiples of 1/4 (e4 = 4*exponent) when they
+  // are, so that x^e = x^(e4/4 integer part) * x^(quarter part) needs only
+  // squarings and square roots; -1 means "use pow()"
+  int n_e4, a_e4;
+};
+
+// Host-side setup (material.cpp)
+void build_matconst(const sc_material& m, const sc_phase_numerics& p, MatConst& out);
+void qu2om_host(const double q[4], double om[9]);  // column-major (3,3)
+
+// Global state arrays of one RVE's material points, structure of arrays:
+// a (3,3) tensor field is 9 planes of N doubles (component i+3j in plane
+// i+3j, column-major as the reference), a slip field is 12 planes.
+struct MpState {
+  int64_t N = 0;
+  double* F = nullptr;       // homogenization_F (input of a response)
+  double* F0 = nullptr;      // phase F0 (converged at end of last increment)
+  double* Fp0 = nullptr;
+  double* Lp0 = nullptr;
+  double* S0 = nullptr;
+  double* xi0 = nullptr;
+  double* gamma0 = nullptr;
+  double* Lp = nullptr;      // Newton guess in, Lp out
+  double* P = nullptr;
+  double* Fp = nullptr;
+  double* S = nullptr;
+  double* Fe = nullptr;
+  double* xi = nullptr;
+  double* gamma = nullptr;
+  int* iters = nullptr;      // (2,N): Newton iterations, state iterations of the last update
+};
+
+// Kernel launchers (constitutive.hip)
+void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_ill, double* d_sub, hipStream_t st,
+                        int* d_broken = nullptr);
+int mp_partial_blocks(int64_t N);
+int mp_sub_planes();
+void launch_pav_partial(const double* P, int64_t N, double* partial, hipStream_t st);
+void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF /*(81,N) SoA*/,
+                       hipStream_t st);
+void launch_mp_forward(const MpState& s, hipStream_t st);
+void launch_mp_init(const MpState& s, const double* d_Fp_init /*(9,N)*/, double xi0, hipStream_t st);
+
+}  // namespace sca

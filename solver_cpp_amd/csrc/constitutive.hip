@@ -1,0 +1,1014 @@
+// Phenopowerlaw material-point update on CDNA4: one lane per material point,
+// FP64, state in structure-of-arrays planes (coalesced across lanes).
+//
+// Follows the reference's per-cell response
+//   homogenization_mechanical_response   homogenization.f90:219-266
+//   phase_mechanical_constitutive        phase_mechanical.f90:1013-1100 (substep cut-back)
+//   integrateStateFPI                    phase_mechanical.f90:611-693
+//   integrateStress                      phase_mechanical.f90:380-604 (Lp Newton)
+//   phase_hooke_SandItsTangents          phase_mechanical_elastic.f90:157-195
+//   phenopowerlaw LpAndItsTangent / dotState / kinetics_sl
+//                                        phase_mechanical_plastic_phenopowerlaw.f90:298-483
+// with Fi = I and Li = 0 (no eigen kinematics are active).  The 9x9 Newton
+// Jacobian I - dLp/dS : dS/dFe : dFe/dLp is assembled directly from its
+// rank-12 structure: with G = Fe^T A and K_s = P_s : C,
+//   J[(a,b),(p,l)] = delta + dt * sum_s c_s P_s(a,b) (G^T K_s)(p,l),
+// c_s = d(gamma_dot_s)/d(tau_s) * sign_s, instead of three 3x3x3x3 products.
+#include "common.h"
+
+namespace sca {
+
+namespace {
+
+// cF <110>{111} slip systems (crystal.f90:41-63): direction, normal.
+__device__ constexpr int SLIP[NSLIP][6] = {
+    {0, 1, -1, 1, 1, 1},   {-1, 0, 1, 1, 1, 1},   {1, -1, 0, 1, 1, 1},
+    {0, -1, -1, -1, -1, 1}, {1, 0, 1, -1, -1, 1},  {-1, 1, 0, -1, -1, 1},
+    {0, -1, 1, 1, -1, -1}, {-1, 0, -1, 1, -1, -1}, {1, 1, 0, 1, -1, -1},
+    {0, 1, 1, -1, 1, -1},  {1, 0, -1, -1, 1, -1},  {-1, -1, 0, -1, 1, -1}};
+
+// sign pattern of P_s(i,j) = d_i n_j
+__device__ __forceinline__ constexpr int psg(int s, int i, int j) { return SLIP[s][i] * SLIP[s][3 + j]; }
+
+// cF slip-slip interaction types (reacting, acting), crystal.f90:623-640
+__device__ constexpr int HTYPE[NSLIP][NSLIP] = {
+    {1, 2, 2, 4, 7, 5, 3, 5, 5, 4, 6, 7}, {2, 1, 2, 7, 4, 5, 6, 4, 7, 5, 3, 5},
+    {2, 2, 1, 5, 5, 3, 6, 7, 4, 7, 6, 4}, {4, 7, 6, 1, 2, 2, 4, 6, 7, 3, 5, 5},
+    {7, 4, 6, 2, 1, 2, 5, 3, 5, 6, 4, 7}, {5, 5, 3, 2, 2, 1, 7, 6, 4, 6, 7, 4},
+    {3, 5, 5, 4, 6, 7, 1, 2, 2, 4, 7, 6}, {6, 4, 7, 5, 3, 5, 2, 1, 2, 7, 4, 6},
+    {6, 7, 4, 7, 6, 4, 2, 2, 1, 5, 5, 3}, {4, 6, 7, 3, 5, 5, 4, 7, 6, 1, 2, 2},
+    {5, 3, 5, 6, 4, 7, 7, 4, 6, 2, 1, 2}, {7, 6, 4, 6, 7, 4, 5, 5, 3, 2, 2, 1}};
+
+// K_s = P_s : C for the cubic stiffness (tr P_s = 0):
+// K_s(m,l) = v C44 (p_ml + p_lm) + v (C11 - C12 - 2 C44) delta_ml p_mm,
+// returned as coefficients of kA = v C44 and kB = v (C11 - C12 - 2 C44).
+__device__ __forceinline__ constexpr int kcoefA(int s, int m, int l) { return psg(s, m, l) + psg(s, l, m); }
+__device__ __forceinline__ constexpr int kcoefB(int s, int m, int l) { return m == l ? psg(s, m, m) : 0; }
+
+struct KernelArgs {
+  MpState s;
+  double dt;
+  int* ill;
+  int* broken;  // per point, may be null
+  double* sub;  // substep scratch, SUB_PLANES planes of N doubles
+};
+
+__constant__ MatConst g_mc;
+
+// ---- small dense helpers (row-major 3x3 in registers: m[3*i+j]) ----------
+__device__ __forceinline__ void mm33(const double* A, const double* B, double* C) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+// math_invert33 (math.f90:511-545); returns false if |det| <= 10*tiny
+__device__ __forceinline__ bool inv33(const double* A, double* R, double* detOut) {
+  R[0] = A[4] * A[8] - A[5] * A[7];
+  R[3] = -A[3] * A[8] + A[5] * A[6];
+  R[6] = A[3] * A[7] - A[4] * A[6];
+  double det = A[0] * R[0] + A[1] * R[3] + A[2] * R[6];
+  if (fabs(det) <= 2.2250738585072014e-308 * 10.0) return false;
+  R[1] = -A[1] * A[8] + A[2] * A[7];
+  R[4] = A[0] * A[8] - A[2] * A[6];
+  R[7] = -A[0] * A[7] + A[1] * A[6];
+  R[2] = A[1] * A[5] - A[2] * A[4];
+  R[5] = -A[0] * A[5] + A[2] * A[3];
+  R[8] = A[0] * A[4] - A[1] * A[3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = R[k] / det;
+  if (detOut) *detOut = det;
+  return true;
+}
+
+__device__ __forceinline__ double det33(const double* m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+__device__ __forceinline__ double norm9(const double* m) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) s += m[k] * m[k];
+  return sqrt(s);
+}
+
+// x^e for x >= 0 with e = e4/4: binary powering of the integer part times
+// sqrt / sqrt-sqrt for the quarter part; libm pow() when e4 < 0
+__device__ __noinline__ double pow_general(double x, double e) { return pow(x, e); }
+__device__ __forceinline__ double powq(double x, int e4, double e) {
+  if (e4 < 0) return pow_general(x, e);
+  double r = 1.0, b = x;
+  int m = e4 >> 2;
+  while (m) {
+    if (m & 1) r *= b;
+    b *= b;
+    m >>= 1;
+  }
+  const int q = e4 & 3;
+  if (q) {
+    const double h = sqrt(x);
+    const double f = sqrt(h);
+    r *= (q == 1) ? f : (q == 2) ? h : h * f;
+  }
+  return r;
+}
+// compile-time quarter exponent E4/4 (E4 >= 0): fully unrolled
+template <int E4>
+__device__ __forceinline__ double powc(double x) {
+  double r = 1.0, b = x;
+#pragma unroll
+  for (int m = E4 >> 2; m; m >>= 1) {
+    if (m & 1) r *= b;
+    b *= b;
+  }
+  constexpr int q = E4 & 3;
+  if (q) {
+    const double h = sqrt(x);
+    const double f = sqrt(h);
+    r *= (q == 1) ? f : (q == 2) ? h : h * f;
+  }
+  return r;
+}
+// exponent dispatch: E4 >= 0 compile-time, E4 == -1 runtime (mc)
+template <int E4>
+__device__ __forceinline__ double powe(double x, int e4rt, double e) {
+  if constexpr (E4 >= 0) return powc<E4>(x);
+  else return powq(x, e4rt, e);
+}
+
+// Hooke (elastic.f90:157-195) with Fi = I, cubic C66:
+// E = (Fe^T Fe - I)/2, S = C66 : E (Voigt strain with engineering shear)
+__device__ __forceinline__ void hooke_S(const double* Fe, double* S, const MatConst& mc) {
+  double E[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      E[3 * i + j] = 0.5 * ((Fe[i] * Fe[j] + Fe[3 + i] * Fe[3 + j] + Fe[6 + i] * Fe[6 + j]) - (i == j ? 1.0 : 0.0));
+  const double e11 = E[0], e22 = E[4], e33 = E[8];
+  const double s11 = mc.C11 * e11 + mc.C12 * e22 + mc.C12 * e33;
+  const double s22 = mc.C12 * e11 + mc.C11 * e22 + mc.C12 * e33;
+  const double s33 = mc.C12 * e11 + mc.C12 * e22 + mc.C11 * e33;
+  const double s23 = mc.C44 * (2.0 * E[7]);
+  const double s13 = mc.C44 * (2.0 * E[6]);
+  const double s12 = mc.C44 * (2.0 * E[1]);
+  S[0] = s11; S[1] = s12; S[2] = s13;
+  S[3] = s12; S[4] = s22; S[5] = s23;
+  S[6] = s13; S[7] = s23; S[8] = s33;
+}
+
+// tau_s = Mp : P_s with Mp = S (Fi = I)
+__device__ __forceinline__ double tau_of(const double* S, int s, double v) {
+  double t = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (psg(s, i, j) != 0) t += S[3 * i + j] * (v * (double)psg(s, i, j));
+  return t;
+}
+
+// kinetics_sl (phenopowerlaw.f90:444-483): tau_pos = t, tau_neg = -t
+template <int NE4>
+__device__ __forceinline__ void kinetics(const double* S, const double* xi, double* dg, double* c,
+                                         const MatConst& mc) {
+#pragma unroll
+  for (int s = 0; s < NSLIP; ++s) {
+    const double tp = tau_of(S, s, mc.v);
+    const double tn = -tp;
+    const double sgn = (tp > tn) ? 1.0 : -1.0;
+    const double tmax = fmax(tp, tn);
+    const double g = sgn * mc.gdot0 * powe<NE4>(tmax / xi[s], mc.n_e4, mc.n);
+    dg[s] = g;
+    if (c) {
+      const double dd = (fabs(g) <= 2.2250738585072014e-308 * 10.0) ? 0.0 : g * mc.n / tmax;
+      c[s] = dd * ((g > 0.0) ? 1.0 : -1.0);  // dLp/dMp = sum c_s P_s (x) P_s
+    }
+  }
+}
+
+// phenopowerlaw_dotState (phenopowerlaw.f90:351-394) for state (xi, gamma)
+template <int NE4, int AE4>
+__device__ __forceinline__ void dot_state(const double* S, const double* st, double* dot, const MatConst& mc) {
+  double dg[NSLIP];
+  kinetics<NE4>(S, st, dg, nullptr, mc);
+#pragma unroll
+  for (int s = 0; s < NSLIP; ++s) dg[s] = fabs(dg[s]);
+#pragma unroll
+  for (int s = 0; s < NSLIP; ++s) {
+    const double x = 1.0 - st[s] / mc.xi_inf;
+    const double left = copysign(powe<AE4>(fabs(x), mc.a_e4, mc.a), x);
+    double hg = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSLIP; ++k) hg += mc.hv[HTYPE[s][k] - 1] * dg[k];
+    dot[s] = mc.h0 * 1.0 * left * hg;
+  }
+#pragma unroll
+  for (int s = 0; s < NSLIP; ++s) dot[NSLIP + s] = dg[s];
+}
+
+// Solve J x = b (9x9, row-major) by LU with partial pivoting; row swaps are
+// done with selects so that every array index stays compile-time constant.
+__device__ __forceinline__ bool solve9_pivot(double* J, double* b) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    int p = k;
+    double mx = fabs(J[9 * k + k]);
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const double v = fabs(J[9 * i + k]);
+      if (v > mx) {
+        mx = v;
+        p = i;
+      }
+    }
+    if (!(mx > 0.0)) return false;
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const bool sw = (p == i);
+#pragma unroll
+      for (int c = k; c < 9; ++c) {
+        const double a = J[9 * k + c], bb = J[9 * i + c];
+        J[9 * k + c] = sw ? bb : a;
+        J[9 * i + c] = sw ? a : bb;
+      }
+      const double a = b[k], bb = b[i];
+      b[k] = sw ? bb : a;
+      b[i] = sw ? a : bb;
+    }
+    const double inv = 1.0 / J[9 * k + k];
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const double l = J[9 * i + k] * inv;
+#pragma unroll
+      for (int c = k + 1; c < 9; ++c) J[9 * i + c] -= l * J[9 * k + c];
+      b[i] -= l * b[k];
+    }
+  }
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+    double s = b[i];
+#pragma unroll
+    for (int c = i + 1; c < 9; ++c) s -= J[9 * i + c] * b[c];
+    b[i] = s / J[9 * i + i];
+  }
+  return true;
+}
+
+// LU without row exchanges; refuses (returns false, J and b untouched in
+// meaning) when a pivot is smaller than 1/100 of its column's largest entry,
+// i.e. where partial pivoting would have to guard against growth.
+__device__ __forceinline__ bool solve9_nopivot(double* J, double* b) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    double mx = 0.0;
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) mx = fmax(mx, fabs(J[9 * i + k]));
+    const double piv = J[9 * k + k];
+    if (!(fabs(piv) * 100.0 >= mx) || piv == 0.0) return false;
+    const double inv = 1.0 / piv;
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const double l = J[9 * i + k] * inv;
+#pragma unroll
+      for (int c = k + 1; c < 9; ++c) J[9 * i + c] -= l * J[9 * k + c];
+      b[i] -= l * b[k];
+    }
+  }
+#pragma unroll
+  for (int i = 8; i >= 0; --i) {
+    double s = b[i];
+#pragma unroll
+    for (int c = i + 1; c < 9; ++c) s -= J[9 * i + c] * b[c];
+    b[i] = s / J[9 * i + i];
+  }
+  return true;
+}
+
+// Column-major plane q = i+3j of a (3,3) field <-> row-major register 3*i+j
+__device__ __forceinline__ int rm(int q) { return 3 * (q % 3) + q / 3; }
+
+__device__ __forceinline__ void load33(const double* base, int64_t N, int64_t c, double* m) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[rm(q)] = base[q * N + c];
+}
+__device__ __forceinline__ void store33(double* base, int64_t N, int64_t c, const double* m) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) base[q * N + c] = m[rm(q)];
+}
+
+// J = I + dt sum_s c_s vec(P_s) vec(G^T K_s)^T with G = Fe^T A
+__device__ __forceinline__ void assemble_jacobian(const double* Fe, const double* A, const double* c, double dt,
+                                                  double* J, const MatConst& mc) {
+  double G[9];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) G[3 * m + p] = Fe[m] * A[p] + Fe[3 + m] * A[3 + p] + Fe[6 + m] * A[6 + p];
+  const double kA = mc.v * mc.C44, kB = mc.v * (mc.C11 - mc.C12 - 2.0 * mc.C44);
+#pragma unroll
+  for (int k = 0; k < 81; ++k) J[k] = (k % 10 == 0) ? 1.0 : 0.0;
+#pragma unroll
+  for (int s = 0; s < NSLIP; ++s) {
+    double W[9];  // W_s = G^T K_s
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        double w = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          if (kcoefA(s, m, l) != 0) w += G[3 * m + p] * (kA * (double)kcoefA(s, m, l));
+          if (kcoefB(s, m, l) != 0) w += G[3 * m + p] * (kB * (double)kcoefB(s, m, l));
+        }
+        W[3 * p + l] = w;
+      }
+    const double f = dt * c[s] * mc.v;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (psg(s, i, j) != 0) {
+          const double fs = f * (double)psg(s, i, j);
+#pragma unroll
+          for (int q = 0; q < 9; ++q) J[9 * (3 * i + j) + q] += fs * W[q];
+        }
+  }
+}
+
+// Per-point output / scratch addressing
+struct PointIO {
+  const MpState* s;
+  int64_t N, c;
+};
+
+// integrateStress (phase_mechanical.f90:380-604); returns broken.  Uses the
+// state xi, updates S and Lp in registers and writes P, Fp, Fe of a
+// successful integration straight to memory (they are read again only as
+// results or as the start of a following substep).
+template <int NE4>
+__device__ __forceinline__ bool integrate_stress(const double* F, const double* Fp0_src, double dt, const double* xi,
+                                                 double* Lp, double* S, int& newton, const PointIO& io,
+                                                 const MatConst& mc) {
+  const int64_t N = io.N, c = io.c;
+  double A[9];
+  {
+    double Fp0[9], invFp0[9];
+    load33(Fp0_src, N, c, Fp0);
+    if (!inv33(Fp0, invFp0, nullptr)) return true;
+    mm33(F, invFp0, A);
+  }
+  double Lpg_old[9], dLp[9];
+  double res_old_norm = 0.0;
+  double step = 1.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    Lpg_old[k] = Lp[k];
+    dLp[k] = 0.0;
+  }
+  double B[9];
+  int it = 0;
+  bool redo = false;  // repeat this iteration's solve with row pivoting
+  for (;;) {
+    if (!redo) {
+      it++;
+      if (it > mc.N_iter_Lp_max) return true;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0 ? 1.0 : 0.0) - dt * Lp[k];
+    double Fe[9];
+    mm33(A, B, Fe);  // Fe = A B invFi_new, invFi_new = I
+    hooke_S(Fe, S, mc);
+    double dg[NSLIP], c[NSLIP];
+    kinetics<NE4>(S, xi, dg, c, mc);
+    double res[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) res[k] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSLIP; ++s)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          if (psg(s, i, j) != 0) res[3 * i + j] = res[3 * i + j] + dg[s] * (mc.v * (double)psg(s, i, j));
+    // res holds Lp_constitutive here
+    const double nLpc = norm9(res);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) res[k] = Lp[k] - res[k];
+    if (!redo) {
+      const double nres = norm9(res);
+      const double atol = fmax(mc.eps_rel_Lp * fmax(norm9(Lp), nLpc), mc.eps_abs_Lp);
+      bool nan = false;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) nan = nan || isnan(res[k]);
+      if (nan) return true;
+      if (nres < atol) break;
+      if (it == 1 || nres < res_old_norm) {
+        res_old_norm = nres;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Lpg_old[k] = Lp[k];
+        step = 1.0;
+      } else {
+        step = mc.r_linesearch_Lp * step;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Lp[k] = Lpg_old[k] + dLp[k] * step;
+        continue;
+      }
+    }
+    double J[81];
+    assemble_jacobian(Fe, A, c, dt, J, mc);
+    const bool solved = redo ? solve9_pivot(J, res) : solve9_nopivot(J, res);
+    if (!solved) {
+      if (redo) return true;
+      redo = true;
+      continue;
+    }
+    redo = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      dLp[k] = -res[k];
+      Lp[k] = Lp[k] + dLp[k] * step;
+    }
+  }
+  newton += it;
+  double invFp_new[9], Fpn[9];
+  {
+    double Fp0[9], invFp0[9];
+    load33(Fp0_src, N, c, Fp0);
+    inv33(Fp0, invFp0, nullptr);
+    mm33(invFp0, B, invFp_new);
+  }
+  if (!inv33(invFp_new, Fpn, nullptr)) return true;
+  double T1[9];
+  mm33(A, B, T1);  // F invFp_new (= Fe, invFi_new = I)
+  store33(io.s->Fe, N, c, T1);
+  {
+    double iFt[9], T2[9], P[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) iFt[3 * i + j] = invFp_new[3 * j + i];
+    mm33(S, iFt, T2);
+    mm33(T1, T2, P);
+    store33(io.s->P, N, c, P);
+  }
+  const double d = pow(det33(Fpn), 1.0 / 3.0);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Fpn[k] = Fpn[k] / d;
+  store33(io.s->Fp, N, c, Fpn);
+  return false;
+}
+
+}  // namespace
+
+// Substep scratch layout (per point, planes of N doubles): F0 (9), Lp0 (9),
+// Fp0 (9), state0 (24) of the current substep start, and F of the substep
+// being integrated (9).  Touched only when the first full step was cut back.
+constexpr int SUB_F0 = 0, SUB_LP0 = 9, SUB_FP0 = 18, SUB_ST0 = 27, SUB_FLOC = 51, SUB_PLANES = 60;
+
+// One residual evaluation's material-point loop.
+template <int NE4, int AE4>
+__global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
+  const MatConst& mc = g_mc;
+  const MpState& s = a.s;
+  const int64_t N = s.N;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // FPI history (dotState_last(:,1), its difference to dotState_last(:,2))
+  // and the accumulated-shear half of the state stay in LDS while the Lp
+  // Newton iteration needs the registers
+  __shared__ double h_last1[NSTATE][64];
+  __shared__ double h_d12[NSTATE][64];
+  __shared__ double h_gam[NSLIP][64];
+  const int t = threadIdx.x;
+  if (c >= N) return;
+  const PointIO io{&s, N, c};
+  double* sub = a.sub;
+  double Lp[9], S[9];
+  load33(s.Lp, N, c, Lp);
+  load33(s.S0, N, c, S);  // phase_restore: S = S0, Fp = Fp0, state = state0
+  double xi[NSLIP];
+#pragma unroll
+  for (int k = 0; k < NSLIP; ++k) {
+    xi[k] = s.xi0[k * N + c];
+    h_gam[k][t] = s.gamma0[k * N + c];
+  }
+  // substep start values live in memory: the increment's converged values
+  // (F0, Lp0, Fp0, state0) until a partial substep succeeded, then the
+  // substep scratch planes
+  bool in_sub = false;
+  int newton = 0, fpi = 0;
+  auto st0 = [&](int k) -> double {
+    return in_sub ? sub[(SUB_ST0 + k) * N + c] : (k < NSLIP ? s.xi0[k * N + c] : s.gamma0[(k - NSLIP) * N + c]);
+  };
+
+  // F of the (sub)step: F0l + step (F - F0), phase_mechanical.f90:1093-1096
+  auto load_Floc = [&](double st_, double* Fl) {
+    double F[9], F0l[9], F0p[9];
+    load33(s.F, N, c, F);
+    load33(in_sub ? sub + SUB_F0 * N : s.F0, N, c, F0l);
+    load33(s.F0, N, c, F0p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fl[k] = F0l[k] + st_ * (F[k] - F0p[k]);
+  };
+  double stepFrac = 0.0, step = 1.0 / mc.r_cutback;
+  bool converged = false, todo = true;
+  while (todo) {
+    if (converged) {
+      stepFrac = stepFrac + step;
+      step = fmin(1.0 - stepFrac, mc.r_increase * step);
+      todo = step > 0.0;
+      if (todo) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          sub[(SUB_F0 + k) * N + c] = sub[(SUB_FLOC + k) * N + c];
+          sub[(SUB_LP0 + k) * N + c] = Lp[rm(k)];
+          sub[(SUB_FP0 + k) * N + c] = s.Fp[k * N + c];
+        }
+#pragma unroll
+        for (int k = 0; k < NSLIP; ++k) {
+          sub[(SUB_ST0 + k) * N + c] = xi[k];
+          sub[(SUB_ST0 + NSLIP + k) * N + c] = h_gam[k][t];
+        }
+        in_sub = true;
+      }
+    } else {
+      step = mc.r_cutback * step;
+      load33(s.S0, N, c, S);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s.Fp[k * N + c] = in_sub ? sub[(SUB_FP0 + k) * N + c] : s.Fp0[k * N + c];
+      if (step < 1.0) load33(in_sub ? sub + SUB_LP0 * N : s.Lp0, N, c, Lp);
+#pragma unroll
+      for (int k = 0; k < NSLIP; ++k) {
+        xi[k] = st0(k);
+        h_gam[k][t] = st0(NSLIP + k);
+      }
+      todo = step > mc.r_cutback_min;
+    }
+    if (todo) {
+      if (step < 1.0) {  // keep this substep's F for the start of the next one
+        double Fl[9];
+        load_Floc(step, Fl);
+        store33(sub + SUB_FLOC * N, N, c, Fl);
+      }
+      const double sdt = step * a.dt;
+      // integrateStateFPI (phase_mechanical.f90:611-693)
+      bool broken = true;
+      double dot[NSTATE];
+      {
+        double stt[NSTATE];
+#pragma unroll
+        for (int k = 0; k < NSLIP; ++k) {
+          stt[k] = xi[k];
+          stt[NSLIP + k] = h_gam[k][t];
+        }
+        dot_state<NE4, AE4>(S, stt, dot, mc);
+      }
+      bool nan = false;
+#pragma unroll
+      for (int k = 0; k < NSTATE; ++k) nan = nan || isnan(dot[k]);
+      if (!nan) {
+#pragma unroll
+        for (int k = 0; k < NSLIP; ++k) {
+          xi[k] = st0(k) + dot[k] * sdt;
+          h_gam[k][t] = st0(NSLIP + k) + dot[NSLIP + k] * sdt;
+        }
+        for (int it = 1; it <= mc.N_iter_state_max; ++it) {
+          fpi++;
+#pragma unroll
+          for (int k = 0; k < NSTATE; ++k) {
+            const double l2 = (it > 1) ? h_last1[k][t] : 0.0;
+            h_d12[k][t] = dot[k] - l2;  // last1 - last2 after the shift
+            h_last1[k][t] = dot[k];
+          }
+          double Floc[9];
+          load_Floc(step, Floc);
+          broken = integrate_stress<NE4>(Floc, in_sub ? sub + SUB_FP0 * N : s.Fp0, sdt, xi, Lp, S, newton, io, mc);
+          if (broken) break;
+          {
+            double stt[NSTATE];
+#pragma unroll
+            for (int k = 0; k < NSLIP; ++k) {
+              stt[k] = xi[k];
+              stt[NSLIP + k] = h_gam[k][t];
+            }
+            dot_state<NE4, AE4>(S, stt, dot, mc);
+          }
+          bool nn = false;
+#pragma unroll
+          for (int k = 0; k < NSTATE; ++k) nn = nn || isnan(dot[k]);
+          if (nn) break;
+          double dp12 = 0.0, dp22 = 0.0, dp01 = 0.0;
+#pragma unroll
+          for (int k = 0; k < NSTATE; ++k) {
+            const double l1 = h_last1[k][t], d12 = h_d12[k][t];
+            dp12 += (dot[k] - l1) * d12;
+            dp22 += d12 * d12;
+            dp01 += dot[k] * l1;
+          }
+          const double zeta =
+              (fmin(dp01, dp12) < 0.0 && dp22 > 0.0) ? 0.75 + 0.25 * tanh(2.0 + 4.0 * dp12 / dp22) : 1.0;
+          bool conv = true;
+#pragma unroll
+          for (int k = 0; k < NSTATE; ++k) {
+            const double l1 = h_last1[k][t];
+            // the damped rate replaces dotState: it is dotState_last(:,1) of
+            // the next iteration (phase_mechanical.f90:653-655)
+            const double dk = dot[k] * zeta + l1 * (1.0 - zeta);
+            dot[k] = dk;
+            const double cur = (k < NSLIP) ? xi[k] : h_gam[k - NSLIP][t];
+            const double r = cur - st0(k) - dk * sdt;
+            const double nw = cur - r;
+            if (k < NSLIP) xi[k] = nw;
+            else h_gam[k - NSLIP][t] = nw;
+            const double at = (k < NSLIP) ? mc.atol_xi : mc.atol_gamma;
+            conv = conv && (fabs(r) <= fmax(at, mc.eps_rel_state * fabs(nw)));
+          }
+          if (conv) {
+            broken = false;
+            break;
+          }
+        }
+      }
+      converged = !broken;
+    }
+  }
+  store33(s.Lp, N, c, Lp);
+  store33(s.S, N, c, S);
+#pragma unroll
+  for (int k = 0; k < NSLIP; ++k) {
+    s.xi[k * N + c] = xi[k];
+    s.gamma[k * N + c] = h_gam[k][t];
+  }
+  if (s.iters) {
+    s.iters[c] = newton;
+    s.iters[N + c] = fpi;
+  }
+  if (a.broken) a.broken[c] = converged ? 0 : 1;
+  if (!converged) atomicOr(a.ill, 1);
+}
+
+// Block partial sums of P for P_av in a fixed-order tree (deterministic):
+// block b sums cells [b*256, b*256+256) of each of the 9 planes.
+__global__ void __launch_bounds__(256) pav_partial_kernel(const double* P, int64_t N, double* partial) {
+  __shared__ double red[256];
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  for (int q = 0; q < 9; ++q) {
+    red[threadIdx.x] = (c < N) ? P[q * N + c] : 0.0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Consistent tangent dPdF (phase_mechanical_dPdF, phase_mechanical.f90:1137-1258)
+// with Fi = I, dLi/dS = 0 (so dFi/dS = 0), using the rank-12 structure of
+// dLp/dS = sum_s c_s P_s (x) P_s.  Output: 81 planes of N doubles, plane
+// index i+3j+9k+27l for dPdF(i,j,k,l) (column-major as the reference).
+// ---------------------------------------------------------------------------
+namespace {
+struct TangentArgs {
+  MpState s;
+  double dt;
+  double* dPdF;
+  double C[81];  // C(i,j,k,l) at 27i+9j+3k+l (row-major)
+};
+
+__device__ __forceinline__ double Cijkl(const TangentArgs& a, int i, int j, int k, int l) {
+  return a.C[27 * i + 9 * j + 3 * k + l];
+}
+}  // namespace
+
+__global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
+  const MatConst& mc = g_mc;
+  const MpState& s = a.s;
+  const int64_t N = s.N;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const double dt = a.dt;
+  double F[9], Fp[9], Fp0[9], S[9], Fe[9], xi[NSLIP];
+  load33(s.F, N, c, F);
+  load33(s.Fp, N, c, Fp);
+  load33(s.Fp0, N, c, Fp0);
+  load33(s.S, N, c, S);
+  load33(s.Fe, N, c, Fe);
+#pragma unroll
+  for (int k = 0; k < NSLIP; ++k) xi[k] = s.xi[k * N + c];
+  double invFp[9], invFp0[9];
+  if (!inv33(Fp, invFp, nullptr)) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) invFp[k] = 0.0;
+  }
+  if (!inv33(Fp0, invFp0, nullptr)) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) invFp0[k] = 0.0;
+  }
+  double dg[NSLIP], cs[NSLIP];
+  kinetics<-1>(S, xi, dg, cs, mc);
+
+  // rhs(p,o,:,:) = Fe C(p,o,:,:) invFp^T  -> R[(p,o)][(k,l)]
+  double R[81];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      double FC[9];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          FC[3 * k + q] = Fe[3 * k] * Cijkl(a, p, o, 0, q) + Fe[3 * k + 1] * Cijkl(a, p, o, 1, q) +
+                          Fe[3 * k + 2] * Cijkl(a, p, o, 2, q);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          R[9 * (3 * p + o) + 3 * k + l] =
+              FC[3 * k] * invFp[3 * l] + FC[3 * k + 1] * invFp[3 * l + 1] + FC[3 * k + 2] * invFp[3 * l + 2];
+    }
+  // L = I + dt sum_s c_s vec(U_s) vec(P_s)^T, U_s = C : (H P_s), H = Fe^T F invFp0
+  double T2[9], H[9];
+  mm33(F, invFp0, T2);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) H[3 * i + j] = Fe[i] * T2[j] + Fe[3 + i] * T2[3 + j] + Fe[6 + i] * T2[6 + j];
+  double L[81];
+#pragma unroll
+  for (int k = 0; k < 81; ++k) L[k] = (k % 10 == 0) ? 1.0 : 0.0;
+#pragma unroll
+  for (int sl = 0; sl < NSLIP; ++sl) {
+    double Z[9];  // H P_s
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          if (psg(sl, m, l) != 0) t += H[3 * i + m] * (mc.v * (double)psg(sl, m, l));
+        Z[3 * i + l] = t;
+      }
+    double U[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) t += Cijkl(a, i, j, m, l) * Z[3 * m + l];
+        U[3 * i + j] = t;
+      }
+    const double f = dt * cs[sl];
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+          if (psg(sl, p, o) != 0) L[9 * q + 3 * p + o] += f * U[q] * (mc.v * (double)psg(sl, p, o));
+  }
+  // dSdF = L^-1 R: LU of L then 9 right-hand sides
+  {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      int p = k;
+      double mx = fabs(L[9 * k + k]);
+#pragma unroll
+      for (int i = k + 1; i < 9; ++i) {
+        const double v = fabs(L[9 * i + k]);
+        if (v > mx) {
+          mx = v;
+          p = i;
+        }
+      }
+#pragma unroll
+      for (int i = k + 1; i < 9; ++i) {
+        const bool sw = (p == i);
+#pragma unroll
+        for (int cc = 0; cc < 9; ++cc) {
+          const double x = L[9 * k + cc], y = L[9 * i + cc];
+          L[9 * k + cc] = sw ? y : x;
+          L[9 * i + cc] = sw ? x : y;
+          const double u = R[9 * k + cc], w = R[9 * i + cc];
+          R[9 * k + cc] = sw ? w : u;
+          R[9 * i + cc] = sw ? u : w;
+        }
+      }
+      const double inv = 1.0 / L[9 * k + k];
+#pragma unroll
+      for (int i = k + 1; i < 9; ++i) {
+        const double l = L[9 * i + k] * inv;
+#pragma unroll
+        for (int cc = k + 1; cc < 9; ++cc) L[9 * i + cc] -= l * L[9 * k + cc];
+#pragma unroll
+        for (int cc = 0; cc < 9; ++cc) R[9 * i + cc] -= l * R[9 * k + cc];
+      }
+    }
+#pragma unroll
+    for (int i = 8; i >= 0; --i)
+#pragma unroll
+      for (int cc = 0; cc < 9; ++cc) {
+        double t = R[9 * i + cc];
+#pragma unroll
+        for (int j = i + 1; j < 9; ++j) t -= L[9 * i + j] * R[9 * j + cc];
+        R[9 * i + cc] = t / L[9 * i + i];
+      }
+  }
+  // R now holds dSdF[(i,j)][(p,o)].
+  // V_s(p,o) = P_s : dSdF(:,:,p,o); dFpinvdF(:,:,p,o) = -dt sum_s c_s V_s(p,o) (invFp0 P_s)
+  double D[81];  // dFpinvdF[(k,l)][(p,o)]
+#pragma unroll
+  for (int k = 0; k < 81; ++k) D[k] = 0.0;
+#pragma unroll
+  for (int sl = 0; sl < NSLIP; ++sl) {
+    double V[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          if (psg(sl, i, j) != 0) t += (mc.v * (double)psg(sl, i, j)) * R[9 * (3 * i + j) + q];
+      V[q] = t;
+    }
+    double Q[9];  // invFp0 P_s
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          if (psg(sl, m, l) != 0) t += invFp0[3 * k + m] * (mc.v * (double)psg(sl, m, l));
+        Q[3 * k + l] = t;
+      }
+    const double f = -dt * cs[sl];
+#pragma unroll
+    for (int kl = 0; kl < 9; ++kl)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) D[9 * kl + q] += f * Q[kl] * V[q];
+  }
+  // assemble dPdF
+  double SiT[9], iFt[9], FiFp[9], u3[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) iFt[3 * i + j] = invFp[3 * j + i];
+  mm33(S, iFt, SiT);        // S invFp^T
+  mm33(F, invFp, FiFp);     // F invFp
+  mm33(FiFp, S, u3);        // F invFp S
+  double d0[9], tmp[9];
+  mm33(invFp, SiT, tmp);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d0[3 * i + j] = tmp[3 * j + i];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      const int q = 3 * p + o;
+      double dF[9], dS[9];
+#pragma unroll
+      for (int kl = 0; kl < 9; ++kl) {
+        dF[kl] = D[9 * kl + q];
+        dS[kl] = R[9 * kl + q];
+      }
+      double a1[9], a2[9], a3[9], t1[9], t2[9], dFt[9];
+      mm33(F, dF, t1);
+      mm33(t1, SiT, a1);
+      mm33(FiFp, dS, t2);
+      mm33(t2, iFt, a2);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dFt[3 * i + j] = dF[3 * j + i];
+      mm33(u3, dFt, a3);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          double v = (k == p) ? d0[3 * l + o] : 0.0;  // dPdF(p,l,p,o) = d0(l,o)
+          v = v + a1[3 * k + l] + a2[3 * k + l] + a3[3 * k + l];
+          // plane index of dPdF(k,l,p,o): k + 3l + 9p + 27o
+          a.dPdF[(int64_t)(k + 3 * l + 9 * p + 27 * o) * N + c] = v;
+        }
+    }
+}
+
+// f_materialpoint_forward: state0 <- state (phase_mechanical.f90:992-1007)
+__global__ void mp_forward_kernel(MpState s) {
+  const int64_t N = s.N;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    s.F0[q * N + c] = s.F[q * N + c];
+    s.Fp0[q * N + c] = s.Fp[q * N + c];
+    s.Lp0[q * N + c] = s.Lp[q * N + c];
+    s.S0[q * N + c] = s.S[q * N + c];
+  }
+#pragma unroll
+  for (int k = 0; k < NSLIP; ++k) {
+    s.xi0[k * N + c] = s.xi[k * N + c];
+    s.gamma0[k * N + c] = s.gamma[k * N + c];
+  }
+}
+
+// mechanical_init (phase_mechanical.f90:253-262) + phenopowerlaw state init
+__global__ void mp_init_kernel(MpState s, const double* Fp_init, double xi_0) {
+  const int64_t N = s.N;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const double id = (q % 4 == 0) ? 1.0 : 0.0;
+    const double fp = Fp_init[q * N + c];
+    s.F[q * N + c] = id;
+    s.F0[q * N + c] = id;
+    s.Fp[q * N + c] = fp;
+    s.Fp0[q * N + c] = fp;
+    s.Lp[q * N + c] = 0.0;
+    s.Lp0[q * N + c] = 0.0;
+    s.S[q * N + c] = 0.0;
+    s.S0[q * N + c] = 0.0;
+    s.P[q * N + c] = 0.0;
+  }
+  // Fe = V_e0 Fp^T = Fp^T: plane i+3j of Fe is plane j+3i of Fp
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s.Fe[(i + 3 * j) * N + c] = Fp_init[(j + 3 * i) * N + c];
+#pragma unroll
+  for (int k = 0; k < NSLIP; ++k) {
+    s.xi[k * N + c] = xi_0;
+    s.xi0[k * N + c] = xi_0;
+    s.gamma[k * N + c] = 0.0;
+    s.gamma0[k * N + c] = 0.0;
+  }
+}
+
+// ---- launchers ---------------------------------------------------------------
+int mp_partial_blocks(int64_t N) { return (int)((N + 255) / 256); }
+int mp_sub_planes() { return SUB_PLANES; }
+
+void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_ill, double* d_sub, hipStream_t st,
+                        int* d_broken) {
+  SCA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mc), &mc, sizeof(MatConst), 0, hipMemcpyHostToDevice, st));
+  KernelArgs a{s, dt, d_ill, d_broken, d_sub};
+  const int blocks = (int)((s.N + 63) / 64);
+  // exponents of the example material (n_sl = 20, a_sl = 2.25) are compiled
+  // in; any other material runs the runtime-exponent instantiation
+  if (mc.n_e4 == 80 && mc.a_e4 == 9)
+    hipLaunchKernelGGL((mp_response_kernel<80, 9>), dim3(blocks), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((mp_response_kernel<-1, -1>), dim3(blocks), dim3(64), 0, st, a);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_pav_partial(const double* P, int64_t N, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(pav_partial_kernel, dim3(mp_partial_blocks(N)), dim3(256), 0, st, P, N, partial);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF, hipStream_t st) {
+  SCA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mc), &mc, sizeof(MatConst), 0, hipMemcpyHostToDevice, st));
+  TangentArgs a;
+  a.s = s;
+  a.dt = dt;
+  a.dPdF = dPdF;
+  // cubic stiffness tensor from C11, C12, C44 (math_Voigt66to3333_stiffness)
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+          double v = 0.0;
+          if (i == j && k == l) v = (i == k) ? mc.C11 : mc.C12;
+          else if ((i == k && j == l) || (i == l && j == k)) v = mc.C44;
+          a.C[27 * i + 9 * j + 3 * k + l] = v;
+        }
+  const int blocks = (int)((s.N + 63) / 64);
+  hipLaunchKernelGGL(mp_tangent_kernel, dim3(blocks), dim3(64), 0, st, a);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_mp_forward(const MpState& s, hipStream_t st) {
+  hipLaunchKernelGGL(mp_forward_kernel, dim3((unsigned)((s.N + 255) / 256)), dim3(256), 0, st, s);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_mp_init(const MpState& s, const double* d_Fp_init, double xi0, hipStream_t st) {
+  hipLaunchKernelGGL(mp_init_kernel, dim3((unsigned)((s.N + 255) / 256)), dim3(256), 0, st, s, d_Fp_init, xi0);
+  SCA_HIP(hipGetLastError());
+}
+
+}  // namespace sca

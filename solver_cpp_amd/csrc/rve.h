@@ -1,0 +1,122 @@
+// One representative volume element resident on the GPU (solver.cpp).
+#pragma once
+#include <algorithm>
+#include <complex>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "spectral.h"
+
+struct sc_rve;
+
+namespace sca {
+
+extern thread_local std::string g_err;
+
+void masked_compliance_host(const double* C, const double q[4], const int mask[9], double* S_out);
+void sym_lstsq(int l, std::vector<double> H, std::vector<double>& b);
+
+// offsets into the small device scratch (doubles); [0, MAX_RED_VEC) holds
+// reduction results
+enum {
+  SM_SUMS = 100,  // 9: P sums of the last response
+  SM_DIV = 110,   // 1: divergence sum
+  SM_AIM = 120,   // 9: DC of the Gamma convolution
+  SM_AV = 130,    // 9: average rate
+  SM_AIM2 = 140,  // 9: forward_field aim
+  SM_FSUM = 150,  // 9: forward_field sums
+  SM_FDIFF = 160, // 9: forward_field correction
+  SM_G81 = 200    // 162: gathered dPdF of argmax / argmin
+};
+
+// PETSc SNESNGMRES (default options, see solver.cpp) on device vectors
+class Ngmres {
+ public:
+  Ngmres(int64_t n, hipStream_t st);
+  ~Ngmres();
+  // residual(x, f): f = F(x); conv(k): 1 converged, -1 diverged, 0 iterating
+  int solve(double* X, int msize, const std::function<void(const double*, double*)>& residual,
+            const std::function<int(int)>& conv, const std::function<void(int)>& set_iter);
+  std::vector<double> reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
+                             const std::vector<const double*>& b);
+  int64_t n;
+  hipStream_t st;
+  double *Fv = nullptr, *XM = nullptr, *FM = nullptr, *XA = nullptr, *FA = nullptr;
+  double *partial = nullptr, *small = nullptr;
+
+ private:
+  std::vector<double*> bufs, hist;
+  double* slot(int k);
+};
+
+class Rve {
+ public:
+  explicit Rve(const sc_rve_desc& d);
+  ~Rve();
+  Rve(const Rve&) = delete;
+  Rve& operator=(const Rve&) = delete;
+
+  void init();
+  void forward(bool cutBack, bool guess, double dt, double dt_old, double t_remaining, const sc_loadstep& stp);
+  sc_solution calculate_solution();
+  int run_loadcase(const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations, int max_incs);
+  void form_residual(const double* Fdev, double* rdev);
+  void constitutive_response(const double* Fdev, double dt);
+  void fetch_response_scalars(bool rotate);
+  void tangent();
+  void update_coords();
+
+  // descriptor and geometry
+  sc_rve_desc desc;
+  std::vector<int> material_at;
+  std::vector<double> orientations;
+  int cells[3];
+  double size[3], scaled[3];
+  int64_t N, K;
+  int nr;
+  double wgt;
+  hipStream_t st = nullptr;
+  MatConst mc;
+  MpState mp;
+  FftPlan<double> fft;
+
+  // device fields (SoA planes)
+  double* X = nullptr;  // F_PETSc
+  double *F_last_inc = nullptr, *F_dot = nullptr;
+  Ngmres* ngm = nullptr;
+  double* spec = nullptr;
+  double* dPdF = nullptr;
+  double* sub = nullptr;
+  double* partial = nullptr;
+  double* small = nullptr;
+  double* d_xi_tab = nullptr;
+  int* d_ill = nullptr;
+  long long* d_minmax_i = nullptr;
+  double *d_u = nullptr, *d_xp = nullptr, *d_xn = nullptr, *d_fsum = nullptr;  // coordinates
+  double* d_n2 = nullptr;  // per-cell |dPdF|^2
+
+  // MechBase state (mech_base.h:75-107), column-major
+  double C_ref[81], C_volAvg[81], C_minMaxAvg[81], C_volAvgLastInc[81], C_minMaxAvgLastInc[81], S[81];
+  double P_av[9], P_aim[9], F_aim_dot[9], F_aim[9], F_aim_last_inc[9];
+  int stress_mask[9];
+  double rot_q[4];
+  double delta_t = 0.0, last_dt = 0.0, last_div_sum = 0.0;
+  int total_iter = 0;
+  double err_BC = 0.0, err_div = 0.0;
+  bool terminally_ill = false;
+  int snes_iter = 0, nfuncs = 0, solve_evals = 0;
+  sc_stats stats;
+
+ private:
+  std::vector<double*> allocs;
+  double* dalloc(size_t n);
+  GammaHost gamma_host(bool do_gamma) const;
+  int converged() const;
+  int snes_ngmres();
+  std::vector<double> reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
+                             const std::vector<const double*>& b, int64_t n);
+};
+
+}  // namespace sca

@@ -1,0 +1,811 @@
+// Device-resident spectral basic solver for one RVE, host side.
+//
+// Mirrors src/grid/mech_solver_basic.cpp (init :19-108, calculate_solution
+// :110-161, forward :163-220, converged :242-284, form_residual :286-339),
+// src/grid/mech_base.cpp (forward_field :213-247, calculate_masked_compliance
+// :249-300, calculate_rate :479-497), src/grid/spectral.cpp
+// (constitutive_response :132-219) and src/grid/damask_grid.cpp
+// (loop_over_loadcase :82-196), with PETSc's SNESNGMRES as the outer solver
+// (PETSc >= 3.12, < 3.21 defaults: m = 30, restart_it = 2, gammaA = gammaC
+// = 2, epsilonB = 0.1, deltaB = 0.9, difference selection and restart, basic
+// line search).  Fields live on the GPU; per residual evaluation the host
+// receives 11 scalars (P_av sums, divergence sum, terminally-ill flag) and
+// per N-GMRES iteration a few dot products.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "rve.h"
+
+namespace sca {
+
+thread_local std::string g_err;
+
+// ---- small host tensor helpers (column-major (3,3) and (3,3,3,3)) ------------
+namespace h {
+inline double& at(double* m, int i, int j) { return m[i + 3 * j]; }
+inline double at(const double* m, int i, int j) { return m[i + 3 * j]; }
+void mul(const double* A, const double* B, double* C) {
+  double t[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += at(A, i, k) * at(B, k, j);
+      t[i + 3 * j] = s;
+    }
+  std::memcpy(C, t, sizeof t);
+}
+void tr(const double* A, double* B) {
+  double t[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) t[i + 3 * j] = at(A, j, i);
+  std::memcpy(B, t, sizeof t);
+}
+// rotate_tensor2, rotations.f90:289-301 (passive R T R^T)
+void rot2(const double q[4], const double* T, double* out) {
+  double R[9], Rt[9], t[9];
+  qu2om_host(q, R);
+  tr(R, Rt);
+  mul(R, T, t);
+  mul(t, Rt, out);
+}
+// rotate_tensor4, rotations.f90:308-328 (passive)
+void rot4(const double q[4], const double* T, double* out) {
+  double R[9];
+  qu2om_host(q, R);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+          double s = 0.0;
+          for (int m = 0; m < 3; ++m)
+            for (int n = 0; n < 3; ++n)
+              for (int o = 0; o < 3; ++o)
+                for (int p = 0; p < 3; ++p)
+                  s += at(R, i, m) * at(R, j, n) * at(R, k, o) * at(R, l, p) * T[m + 3 * n + 9 * o + 27 * p];
+          out[i + 3 * j + 9 * k + 27 * l] = s;
+        }
+}
+// math_mul3333xx33
+void mul3333xx33(const double* A, const double* B, double* out) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int n = 0; n < 3; ++n)
+        for (int m = 0; m < 3; ++m) s += A[i + 3 * j + 9 * m + 27 * n] * B[m + 3 * n];
+      out[i + 3 * j] = s;
+    }
+}
+// LU inverse with partial pivoting (row-major n x n); returns false on a zero pivot
+bool invert(int n, const double* a, double* inv) {
+  std::vector<double> lu(a, a + n * n);
+  std::vector<int> piv(n);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double mx = std::fabs(lu[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(lu[i * n + k]) > mx) {
+        mx = std::fabs(lu[i * n + k]);
+        p = i;
+      }
+    piv[k] = p;
+    if (lu[p * n + k] == 0.0) return false;
+    if (p != k)
+      for (int j = 0; j < n; ++j) std::swap(lu[k * n + j], lu[p * n + j]);
+    const double d = 1.0 / lu[k * n + k];
+    for (int i = k + 1; i < n; ++i) lu[i * n + k] *= d;
+    for (int i = k + 1; i < n; ++i)
+      for (int j = k + 1; j < n; ++j) lu[i * n + j] -= lu[i * n + k] * lu[k * n + j];
+  }
+  std::vector<double> col(n);
+  for (int c = 0; c < n; ++c) {
+    for (int i = 0; i < n; ++i) col[i] = (i == c) ? 1.0 : 0.0;
+    for (int k = 0; k < n; ++k)
+      if (piv[k] != k) std::swap(col[k], col[piv[k]]);
+    for (int i = 1; i < n; ++i)
+      for (int j = 0; j < i; ++j) col[i] -= lu[i * n + j] * col[j];
+    for (int i = n - 1; i >= 0; --i) {
+      for (int j = i + 1; j < n; ++j) col[i] -= lu[i * n + j] * col[j];
+      col[i] /= lu[i * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[i * n + c] = col[i];
+  }
+  return true;
+}
+}  // namespace h
+
+void masked_compliance_host(const double* C, const double q[4], const int mask[9], double* S_out) {
+  bool m1d[9];
+  int nred = 0;
+  for (int col = 0; col < 3; ++col)
+    for (int row = 0; row < 3; ++row) {
+      m1d[col * 3 + row] = !mask[row + 3 * col];
+      nred += m1d[col * 3 + row] ? 1 : 0;
+    }
+  double t99[81] = {0};  // plain map, row-major (p,q), p = 3i+j
+  if (nred > 0) {
+    double rot[81];
+    h::rot4(q, C, rot);
+    double c99[81];
+    for (int p = 0; p < 9; ++p)
+      for (int qq = 0; qq < 9; ++qq) {
+        const int i = p / 3, j = p % 3, k = qq / 3, l = qq % 3;
+        c99[p * 9 + qq] = rot[i + 3 * j + 9 * k + 27 * l];
+      }
+    std::vector<double> cr(nred * nred), sr(nred * nred);
+    int idx = 0;
+    for (int i = 0; i < 9; ++i)
+      for (int j = 0; j < 9; ++j)
+        if (m1d[i] && m1d[j]) {
+          cr[(idx / nred) * nred + idx % nred] = c99[i * 9 + j];
+          ++idx;
+        }
+    if (!h::invert(nred, cr.data(), sr.data())) throw std::runtime_error("Matrix inversion error");
+    idx = 0;
+    for (int i = 0; i < 9; ++i)
+      for (int j = 0; j < 9; ++j)
+        if (m1d[i] && m1d[j]) {
+          t99[i * 9 + j] = sr[(idx / nred) * nred + idx % nred];
+          ++idx;
+        }
+  }
+  for (int p = 0; p < 9; ++p)
+    for (int qq = 0; qq < 9; ++qq) {
+      const int i = p / 3, j = p % 3, k = qq / 3, l = qq % 3;
+      S_out[i + 3 * j + 9 * k + 27 * l] = t99[p * 9 + qq];
+    }
+}
+
+// min-norm least squares of a symmetric l x l system via cyclic Jacobi,
+// singular values <= eps*s_max dropped (LAPACK dgelss, rcond = -1)
+void sym_lstsq(int l, std::vector<double> H, std::vector<double>& b) {
+  std::vector<double> V(l * l, 0.0);
+  for (int i = 0; i < l; ++i) V[i * l + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < l; ++p)
+      for (int q = p + 1; q < l; ++q) off += H[p * l + q] * H[p * l + q];
+    if (off < 1e-300) break;
+    for (int p = 0; p < l; ++p)
+      for (int q = p + 1; q < l; ++q) {
+        const double apq = H[p * l + q];
+        if (apq == 0.0) continue;
+        const double theta = (H[q * l + q] - H[p * l + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < l; ++k) {
+          const double a = H[k * l + p], bb = H[k * l + q];
+          H[k * l + p] = c * a - s * bb;
+          H[k * l + q] = s * a + c * bb;
+        }
+        for (int k = 0; k < l; ++k) {
+          const double a = H[p * l + k], bb = H[q * l + k];
+          H[p * l + k] = c * a - s * bb;
+          H[q * l + k] = s * a + c * bb;
+        }
+        for (int k = 0; k < l; ++k) {
+          const double a = V[k * l + p], bb = V[k * l + q];
+          V[k * l + p] = c * a - s * bb;
+          V[k * l + q] = s * a + c * bb;
+        }
+      }
+  }
+  double smax = 0.0;
+  for (int i = 0; i < l; ++i) smax = std::max(smax, std::fabs(H[i * l + i]));
+  const double thr = std::max(2.220446049250313e-16 * smax, 2.2250738585072014e-308);
+  std::vector<double> x(l, 0.0);
+  for (int i = 0; i < l; ++i) {
+    const double lam = H[i * l + i];
+    if (std::fabs(lam) <= thr) continue;
+    double proj = 0.0;
+    for (int k = 0; k < l; ++k) proj += V[k * l + i] * b[k];
+    for (int k = 0; k < l; ++k) x[k] += V[k * l + i] * proj / lam;
+  }
+  b = x;
+}
+
+// ---------------------------------------------------------------------------
+double* Rve::dalloc(size_t n) {
+  double* p = nullptr;
+  SCA_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)));
+  allocs.push_back(p);
+  return p;
+}
+
+Rve::Rve(const sc_rve_desc& d) {
+  desc = d;
+  for (int p = 0; p < 3; ++p) {
+    cells[p] = d.cells[p];
+    size[p] = d.size[p];
+  }
+  if (cells[0] < 2) throw std::runtime_error("cells[0] must be larger than 1");
+  N = (int64_t)cells[0] * cells[1] * cells[2];
+  nr = cells[0] / 2 + 1;
+  K = (int64_t)nr * cells[1] * cells[2];
+  wgt = std::pow((double)(cells[0] * cells[1] * cells[2]), -1);
+  material_at.assign(d.material_at, d.material_at + N);
+  orientations.assign(d.orientations, d.orientations + 4 * d.n_materials);
+  desc.material_at = material_at.data();
+  desc.orientations = orientations.data();
+  // scaled geometry, mech_base.cpp:11-33
+  {
+    int idx[3] = {0, 1, 2};
+    const int dc = d.grid_num.divergence_correction;
+    if (dc == 1) {
+      std::sort(idx, idx + 3, [&](int a, int b) { return size[a] < size[b]; });
+      for (int j = 0; j < 3; ++j) scaled[j] = size[j] / size[idx[1]];
+    } else if (dc == 2) {
+      double ng[3] = {size[0] / cells[0], size[1] / cells[1], size[2] / cells[2]};
+      std::sort(idx, idx + 3, [&](int a, int b) { return ng[a] < ng[b]; });
+      for (int j = 0; j < 3; ++j) scaled[j] = size[j] / size[idx[1]] * cells[idx[1]];
+    } else {
+      for (int j = 0; j < 3; ++j) scaled[j] = size[j];
+    }
+  }
+  build_matconst(d.material, d.phase_num, mc);
+  SCA_HIP(hipStreamCreate(&st));
+  fft.init(cells);
+  ngm = new Ngmres(9 * N, st);
+  // material point planes
+  mp.N = N;
+  mp.F = dalloc(9 * N);
+  mp.F0 = dalloc(9 * N);
+  mp.Fp0 = dalloc(9 * N);
+  mp.Lp0 = dalloc(9 * N);
+  mp.S0 = dalloc(9 * N);
+  mp.xi0 = dalloc(12 * N);
+  mp.gamma0 = dalloc(12 * N);
+  mp.Lp = dalloc(9 * N);
+  mp.P = dalloc(9 * N);
+  mp.Fp = dalloc(9 * N);
+  mp.S = dalloc(9 * N);
+  mp.Fe = dalloc(9 * N);
+  mp.xi = dalloc(12 * N);
+  mp.gamma = dalloc(12 * N);
+  SCA_HIP(hipMalloc(&mp.iters, 2 * N * sizeof(int)));
+  sub = dalloc((size_t)mp_sub_planes() * N);
+  X = dalloc(9 * N);
+  F_last_inc = dalloc(9 * N);
+  F_dot = dalloc(9 * N);
+  spec = dalloc(2 * 9 * K);
+  dPdF = dalloc(81 * N);
+  partial = dalloc((size_t)std::max<int64_t>(red_blocks() * MAX_RED_VEC, 9 * mp_partial_blocks(N)) +
+                   gamma_blocks(K));
+  small = dalloc(512);
+  d_u = dalloc(3 * N);
+  d_xp = dalloc(3 * N);
+  d_xn = dalloc((size_t)3 * (cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1));
+  d_fsum = dalloc(9);
+  d_n2 = dalloc(N);
+  SCA_HIP(hipMalloc(&d_ill, sizeof(int)));
+  SCA_HIP(hipMalloc(&d_minmax_i, 2 * red_blocks() * sizeof(long long)));
+  if (d.grid_num.derivative != 0) {
+    // xi table in the kernel layout: (3 complex) per k = i + nr*(y + Ny*z)
+    std::vector<double> tab(6 * K);
+    const int Ny = cells[1], Nz = cells[2];
+    const double TAU = 2.0 * M_PI;
+    for (int z = 0; z < Nz; ++z)
+      for (int y = 0; y < Ny; ++y)
+        for (int i = 0; i < nr; ++i) {
+          const int ks[3] = {i, (y > Ny / 2) ? y - Ny : y, (z > Nz / 2) ? z - Nz : z};
+          const int64_t kk = i + (int64_t)nr * (y + (int64_t)Ny * z);
+          for (int p = 0; p < 3; ++p) {
+            double re, im;
+            if (d.grid_num.derivative == 1) {
+              re = 0.0;
+              im = std::sin(TAU * ks[p] / cells[p]) / (2.0 * size[p] / cells[p]);
+            } else {
+              const int p1 = (p + 1) % 3, p2 = (p + 2) % 3;
+              std::complex<double> v = std::complex<double>(std::cos(TAU * ks[p] / cells[p]) - (p == 0 ? 1.0 : -1.0),
+                                                            std::sin(TAU * ks[p] / cells[p])) *
+                                       std::complex<double>(std::cos(TAU * ks[p1] / cells[p1]) + 1.0,
+                                                            std::sin(TAU * ks[p1] / cells[p1])) *
+                                       std::complex<double>(std::cos(TAU * ks[p2] / cells[p2]) + 1.0,
+                                                            std::sin(TAU * ks[p2] / cells[p2])) /
+                                       std::complex<double>(4.0 * size[p] / cells[p], 0.0);
+              re = v.real();
+              im = v.imag();
+            }
+            tab[6 * kk + 2 * p] = re;
+            tab[6 * kk + 2 * p + 1] = im;
+          }
+        }
+    d_xi_tab = dalloc(6 * K);
+    SCA_HIP(hipMemcpy(d_xi_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // mechanical_init: F = I, Fp = O_0, Fe = Fp^T, Lp = S = 0, xi = xi_0, gamma = 0
+  {
+    std::vector<double> fp(9 * N);
+    std::vector<double> om(9 * (size_t)d.n_materials);
+    for (int m = 0; m < d.n_materials; ++m) qu2om_host(&orientations[4 * m], &om[9 * m]);
+    for (int64_t c = 0; c < N; ++c) {
+      const int ma = material_at[c];
+      if (ma < 0 || ma >= d.n_materials) throw std::runtime_error("material_at out of range");
+      for (int q = 0; q < 9; ++q) fp[q * N + c] = om[9 * ma + q];
+    }
+    double* d_fp = dalloc(9 * N);
+    SCA_HIP(hipMemcpy(d_fp, fp.data(), fp.size() * sizeof(double), hipMemcpyHostToDevice));
+    launch_mp_init(mp, d_fp, d.material.xi_0_sl, st);
+    SCA_HIP(hipStreamSynchronize(st));
+    SCA_HIP(hipFree(d_fp));
+    allocs.pop_back();
+  }
+  std::memset(C_ref, 0, sizeof C_ref);
+  std::memset(C_volAvg, 0, sizeof C_volAvg);
+  std::memset(C_minMaxAvg, 0, sizeof C_minMaxAvg);
+  std::memset(C_volAvgLastInc, 0, sizeof C_volAvgLastInc);
+  std::memset(C_minMaxAvgLastInc, 0, sizeof C_minMaxAvgLastInc);
+  std::memset(S, 0, sizeof S);
+  std::memset(P_av, 0, sizeof P_av);
+  std::memset(P_aim, 0, sizeof P_aim);
+  std::memset(F_aim_dot, 0, sizeof F_aim_dot);
+  for (int q = 0; q < 9; ++q) F_aim[q] = F_aim_last_inc[q] = (q % 4 == 0) ? 1.0 : 0.0;
+  for (int q = 0; q < 9; ++q) stress_mask[q] = 0;
+  rot_q[0] = 1.0;
+  rot_q[1] = rot_q[2] = rot_q[3] = 0.0;
+  std::memset(&stats, 0, sizeof stats);
+}
+
+Rve::~Rve() {
+  delete ngm;
+  for (double* p : allocs) (void)hipFree(p);
+  if (mp.iters) (void)hipFree(mp.iters);
+  if (d_ill) (void)hipFree(d_ill);
+  if (d_minmax_i) (void)hipFree(d_minmax_i);
+  fft.release();
+  if (st) (void)hipStreamDestroy(st);
+}
+
+// deterministic device reduction of nvec vectors of length n -> host values
+std::vector<double> Rve::reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
+                                const std::vector<const double*>& b, int64_t n) {
+  const int nvec = (int)ops.size();
+  std::vector<double> out(nvec);
+  for (int v0 = 0; v0 < nvec; v0 += MAX_RED_VEC) {
+    const int nv = std::min(MAX_RED_VEC, nvec - v0);
+    launch_partial_reduce<double>(ops.data() + v0, n, nv, a.data() + v0, b.empty() ? nullptr : b.data() + v0,
+                                  partial, st);
+    launch_final_reduce(partial, nv, red_blocks(), small, st);
+    SCA_HIP(hipMemcpyAsync(out.data() + v0, small, nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    SCA_HIP(hipStreamSynchronize(st));
+  }
+  return out;
+}
+
+// Spectral::constitutive_response (spectral.cpp:132-219) up to the P_av sums
+// on the device; the tangent reductions are deferred to tangent().
+void Rve::constitutive_response(const double* Fdev, double dt) {
+  SCA_HIP(hipMemcpyAsync(mp.F, Fdev, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  SCA_HIP(hipMemsetAsync(d_ill, 0, sizeof(int), st));
+  launch_mp_response(mp, mc, dt, d_ill, sub, st);
+  launch_pav_partial(mp.P, N, partial, st);
+  launch_final_reduce(partial, 9, mp_partial_blocks(N), small + SM_SUMS, st);
+  last_dt = dt;
+}
+
+void Rve::fetch_response_scalars(bool rotate) {
+  double h_s[11];
+  int ill = 0;
+  SCA_HIP(hipMemcpyAsync(h_s, small + SM_SUMS, 9 * sizeof(double), hipMemcpyDeviceToHost, st));
+  SCA_HIP(hipMemcpyAsync(h_s + 9, small + SM_DIV, sizeof(double), hipMemcpyDeviceToHost, st));
+  SCA_HIP(hipMemcpyAsync(&ill, d_ill, sizeof(int), hipMemcpyDeviceToHost, st));
+  SCA_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < 9; ++q) P_av[q] = h_s[q] * wgt;
+  if (rotate) h::rot2(rot_q, P_av, P_av);
+  last_div_sum = h_s[9];
+  if (ill) terminally_ill = true;
+}
+
+// consistent tangent of the last evaluation: C_volAvg, C_minMaxAvg
+// (spectral.cpp:189-217)
+void Rve::tangent() {
+  launch_mp_tangent(mp, mc, last_dt, dPdF, st);
+  std::vector<int> ops(81, 2);
+  std::vector<const double*> a(81);
+  for (int q = 0; q < 81; ++q) a[q] = dPdF + q * N;
+  std::vector<double> sums = reduce(ops, a, {}, N);
+  for (int q = 0; q < 81; ++q) C_volAvg[q] = sums[q] * wgt;
+  int64_t imx = -1, imn = -1;
+  tangent_minmax_select(dPdF, N, d_n2, partial, d_minmax_i, imx, imn, st);
+  double dmax[81], dmin[81];
+  for (int q = 0; q < 81; ++q) {
+    dmax[q] = 0.0;
+    dmin[q] = std::numeric_limits<double>::max();
+  }
+  if (imx >= 0) {
+    launch_gather81(dPdF, N, imx, small + SM_G81, st);
+    SCA_HIP(hipMemcpyAsync(dmax, small + SM_G81, 81 * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  if (imn >= 0) {
+    launch_gather81(dPdF, N, imn, small + SM_G81 + 81, st);
+    SCA_HIP(hipMemcpyAsync(dmin, small + SM_G81 + 81, 81 * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  SCA_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < 81; ++q) C_minMaxAvg[q] = 0.5 * (dmax[q] + dmin[q]);
+}
+
+GammaHost Rve::gamma_host(bool do_gamma) const {
+  GammaHost g;
+  for (int p = 0; p < 3; ++p) {
+    g.cells[p] = cells[p];
+    g.size[p] = size[p];
+    g.rescale[p] = size[p] / scaled[p];
+  }
+  // C_ref (column-major i+3j+9k+27l) -> kernel layout 27l+9n+3m+o for C(l,n,m,o)
+  for (int l = 0; l < 3; ++l)
+    for (int n = 0; n < 3; ++n)
+      for (int m = 0; m < 3; ++m)
+        for (int o = 0; o < 3; ++o) g.C_ref[27 * l + 9 * n + 3 * m + o] = C_ref[l + 3 * n + 9 * m + 27 * o];
+  g.derivative = desc.grid_num.derivative;
+  g.do_gamma = do_gamma ? 1 : 0;
+  return g;
+}
+
+// MechSolverBasic::form_residual, mech_solver_basic.cpp:286-339
+void Rve::form_residual(const double* Fdev, double* rdev) {
+  if (nfuncs == 0 && snes_iter == 0) total_iter = -1;
+  if (total_iter <= snes_iter) total_iter += 1;
+  constitutive_response(Fdev, delta_t);
+  // DC of the convolution: rotated S : P_av, computed on the device
+  double R[9], Rrm[9];
+  qu2om_host(rot_q, R);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Rrm[3 * i + j] = R[i + 3 * j];
+  launch_bc_aim(small + SM_SUMS, S, Rrm, wgt, 1, small + SM_AIM, st);
+  fft.forward(mp.P, spec, 9, st);
+  launch_gamma_div<double>(spec, K, gamma_host(true), small + SM_AIM, d_xi_tab, partial, st);
+  launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
+  fft.backward(spec, rdev, 9, 1.0, st);
+  fetch_response_scalars(true);
+  err_div = std::sqrt(last_div_sum) * wgt;
+  if (cells[0] == 1) err_div = err_div / 2;
+  double dF[9];
+  h::mul3333xx33(S, P_av, dF);
+  for (int q = 0; q < 9; ++q) F_aim[q] = F_aim[q] - dF[q];
+  double e = 0.0;
+  for (int q = 0; q < 9; ++q)
+    if (!stress_mask[q]) e = std::max(e, std::fabs(P_av[q] - P_aim[q]));
+  err_BC = e;
+  nfuncs++;
+  solve_evals++;
+  stats.residual_evaluations++;
+  stats.material_point_updates += N;
+}
+
+// MechSolverBasic::converged, mech_solver_basic.cpp:242-284
+int Rve::converged() const {
+  const auto& gn = desc.grid_num;
+  double maxP = -std::numeric_limits<double>::infinity();
+  for (int q = 0; q < 9; ++q) maxP = std::max(maxP, P_av[q]);
+  const double divTol = std::max(maxP * gn.eps_rel_div, gn.eps_abs_div);
+  const double BCTol = std::max(maxP * gn.eps_rel_P, gn.eps_abs_P);
+  if ((total_iter >= gn.itmin && std::max(err_div / divTol, err_BC / BCTol) < 1.0) || terminally_ill) return 1;
+  if (total_iter >= gn.itmax) return -1;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// SNESNGMRES on device vectors of length n
+// ---------------------------------------------------------------------------
+Ngmres::Ngmres(int64_t n_, hipStream_t st_) : n(n_), st(st_) {
+  auto a = [&](double*& p, size_t m) {
+    SCA_HIP(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(double)));
+    bufs.push_back(p);
+  };
+  a(Fv, n);
+  a(XM, n);
+  a(FM, n);
+  a(XA, n);
+  a(FA, n);
+  a(partial, (size_t)red_blocks() * MAX_RED_VEC);
+  a(small, MAX_RED_VEC);
+}
+Ngmres::~Ngmres() {
+  for (double* p : bufs) (void)hipFree(p);
+  for (double* p : hist) (void)hipFree(p);
+}
+double* Ngmres::slot(int k) {
+  while ((int)hist.size() <= k) {
+    double* p = nullptr;
+    SCA_HIP(hipMalloc(&p, n * sizeof(double)));
+    hist.push_back(p);
+  }
+  return hist[k];
+}
+std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
+                                   const std::vector<const double*>& b) {
+  const int nvec = (int)ops.size();
+  std::vector<double> out(nvec);
+  for (int v0 = 0; v0 < nvec; v0 += MAX_RED_VEC) {
+    const int nv = std::min(MAX_RED_VEC, nvec - v0);
+    launch_partial_reduce<double>(ops.data() + v0, n, nv, a.data() + v0, b.empty() ? nullptr : b.data() + v0,
+                                  partial, st);
+    launch_final_reduce(partial, nv, red_blocks(), small, st);
+    SCA_HIP(hipMemcpyAsync(out.data() + v0, small, nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    SCA_HIP(hipStreamSynchronize(st));
+  }
+  return out;
+}
+
+int Ngmres::solve(double* X, int msize, const std::function<void(const double*, double*)>& residual,
+                  const std::function<int(int)>& conv, const std::function<void(int)>& set_iter) {
+  const int max_its = 50;
+  const double gammaA = 2.0, gammaC = 2.0, epsilonB = 0.1, deltaB = 0.9;
+  const int restart_it = 2;
+  const size_t bytes = n * sizeof(double);
+  auto Fdot = [&](int i) { return slot(2 * i); };
+  auto Xdot = [&](int i) { return slot(2 * i + 1); };
+  std::vector<double> fnorms(msize, 0.0), Q(msize * msize, 0.0);
+  set_iter(0);
+  residual(X, Fv);
+  double fnorm = std::sqrt(reduce({0}, {Fv}, {Fv})[0]);
+  if (std::isnan(fnorm) || std::isinf(fnorm)) return -4;  // SNES_DIVERGED_FNORM_NAN
+  double fminnorm = fnorm;
+  int reason = conv(0);
+  if (reason) return reason > 0 ? 6 : -5;
+  auto update_subspace = [&](int iv, const double* f, double fn, const double* x) {
+    SCA_HIP(hipMemcpyAsync(Fdot(iv), f, bytes, hipMemcpyDeviceToDevice, st));
+    SCA_HIP(hipMemcpyAsync(Xdot(iv), x, bytes, hipMemcpyDeviceToDevice, st));
+    fnorms[iv] = fn;
+  };
+  update_subspace(0, Fv, fnorm, X);
+  int k_restart = 1, l = 1, ivec = 0, restart_count = 0;
+  for (int k = 1; k < max_its + 1; ++k) {
+    launch_axpby(1.0, X, -1.0, Fv, XM, n, st);  // basic line search: XM = X - F
+    residual(XM, FM);
+    // ||FM||^2, <FM, Fdot_i>, <Fdot_ivec, Fdot_i> in one pass
+    std::vector<int> ops(1 + 2 * l, 0);
+    std::vector<const double*> va(1 + 2 * l), vb(1 + 2 * l);
+    va[0] = FM;
+    vb[0] = FM;
+    for (int i = 0; i < l; ++i) {
+      va[1 + i] = FM;
+      vb[1 + i] = Fdot(i);
+      va[1 + l + i] = Fdot(ivec);
+      vb[1 + l + i] = Fdot(i);
+    }
+    const std::vector<double> dots = reduce(ops, va, vb);
+    const double fMnorm = std::sqrt(dots[0]);
+    if (std::isnan(fMnorm) || std::isinf(fMnorm)) return -6;  // SNES_DIVERGED_LINE_SEARCH
+    {  // SNESNGMRESFormCombinedSolution_Private
+      const double nu = fMnorm * fMnorm;
+      std::vector<double> xi(l), beta(l);
+      for (int i = 0; i < l; ++i) {
+        xi[i] = dots[1 + i];
+        beta[i] = dots[1 + l + i];
+        Q[i * msize + ivec] = beta[i];
+        Q[ivec * msize + i] = beta[i];
+      }
+      for (int i = 0; i < l; ++i) beta[i] = nu - xi[i];
+      std::vector<double> H(l * l);
+      for (int j = 0; j < l; ++j)
+        for (int i = 0; i < l; ++i) H[i * l + j] = Q[i * msize + j] - xi[i] - xi[j] + nu;
+      if (l == 1)
+        beta[0] = (H[0] != 0.0) ? beta[0] / H[0] : 0.0;
+      else
+        sym_lstsq(l, H, beta);
+      double alph_total = 0.0;
+      for (int i = 0; i < l; ++i) alph_total += beta[i];
+      std::vector<const double*> xs(l);
+      for (int i = 0; i < l; ++i) xs[i] = Xdot(i);
+      launch_lincomb(XM, 1.0 - alph_total, l, beta.data(), xs.data(), XA, n, st);
+      residual(XA, FA);
+    }
+    if (fminnorm > fMnorm) fminnorm = fMnorm;
+    // ||FA||^2, ||XA - XM||^2, ||Xdot_i - XA||^2 (SNESNGMRESNorms_Private)
+    std::vector<int> ops2(2 + l, 1);
+    std::vector<const double*> wa(2 + l), wb(2 + l);
+    ops2[0] = 0;
+    wa[0] = FA;
+    wb[0] = FA;
+    wa[1] = XA;
+    wb[1] = XM;
+    for (int i = 0; i < l; ++i) {
+      wa[2 + i] = Xdot(i);
+      wb[2 + i] = XA;
+    }
+    const std::vector<double> nrm = reduce(ops2, wa, wb);
+    const double fAnorm = std::sqrt(nrm[0]);
+    const double dnorm = std::sqrt(nrm[1]);
+    double dminnorm = -1.0;
+    for (int i = 0; i < l; ++i) {
+      const double d = std::sqrt(nrm[2 + i]);
+      if (d < dminnorm || dminnorm < 0.0) dminnorm = d;
+    }
+    if (std::isnan(fnorm) || std::isinf(fnorm)) return -4;
+    // SNESNGMRESSelect_Private / SelectRestart_Private (difference)
+    bool selectA = fAnorm < gammaA * fminnorm;
+    if (!(epsilonB * dnorm < dminnorm || std::sqrt(fnorm) < deltaB * std::sqrt(fminnorm))) selectA = false;
+    bool selectRestart = false;
+    if (epsilonB * dnorm > dminnorm && fAnorm > deltaB * fminnorm && l > 0) selectRestart = true;
+    if (fAnorm > gammaC * fminnorm) selectRestart = true;
+    restart_count = selectRestart ? restart_count + 1 : 0;
+    ivec = k_restart % msize;
+    const bool do_restart = restart_count >= restart_it;
+    if (do_restart) update_subspace(0, FM, fMnorm, XM);
+    if (selectA) {
+      fnorm = fAnorm;
+      SCA_HIP(hipMemcpyAsync(Fv, FA, bytes, hipMemcpyDeviceToDevice, st));
+      SCA_HIP(hipMemcpyAsync(X, XA, bytes, hipMemcpyDeviceToDevice, st));
+    } else {
+      fnorm = fMnorm;
+      SCA_HIP(hipMemcpyAsync(Fv, FM, bytes, hipMemcpyDeviceToDevice, st));
+      SCA_HIP(hipMemcpyAsync(X, XM, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    if (do_restart) {
+      restart_count = 0;
+      k_restart = 1;
+      l = 1;
+      ivec = 0;
+    } else {
+      if (l < msize) l++;
+      k_restart++;
+      if (fminnorm > fnorm) fminnorm = fnorm;
+      update_subspace(ivec, Fv, fnorm, X);
+    }
+    set_iter(k);
+    reason = conv(k);
+    if (reason) return reason > 0 ? 6 : -5;
+  }
+  return -5;  // SNES_DIVERGED_MAX_IT
+}
+
+int Rve::snes_ngmres() {
+  const int msize = desc.grid_num.ngmres_m > 0 ? desc.grid_num.ngmres_m : 30;
+  snes_iter = 0;
+  nfuncs = 0;
+  solve_evals = 0;
+  return ngm->solve(
+      X, msize, [this](const double* x, double* f) { form_residual(x, f); }, [this](int) { return converged(); },
+      [this](int k) { snes_iter = k; });
+}
+
+// MechSolverBasic::update_coords -> base_update_coords (mech_base.cpp:45-154):
+// cell centres and nodes of the deformed configuration from the average F
+// and the periodic displacement fluctuation u = FFT^-1(F_hat xi / |xi|^2)
+void Rve::update_coords() {
+  fft.forward(X, spec, 9, st);
+  launch_coords_uhat(spec, K, gamma_host(false), d_xi_tab, d_fsum, st);
+  fft.backward(spec, d_u, 3, wgt, st);
+  launch_coords_nodes(d_u, d_fsum, cells, size, d_xn, st);
+  launch_coords_points(d_u, d_fsum, cells, size, d_xp, st);
+}
+
+void Rve::init() {
+  std::vector<double> eye(9 * N);
+  for (int q = 0; q < 9; ++q)
+    for (int64_t c = 0; c < N; ++c) eye[q * N + c] = (q % 4 == 0) ? 1.0 : 0.0;
+  SCA_HIP(hipMemcpy(F_last_inc, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice));
+  SCA_HIP(hipMemcpy(X, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice));
+  update_coords();
+  constitutive_response(X, 0.0);
+  fetch_response_scalars(false);
+  tangent();
+  for (int q = 0; q < 81; ++q) C_ref[q] = C_minMaxAvg[q] / wgt;
+}
+
+void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_remaining, const sc_loadstep& stp) {
+  if (cutBack) {
+    std::memcpy(C_volAvg, C_volAvgLastInc, sizeof C_volAvg);
+    std::memcpy(C_minMaxAvg, C_minMaxAvgLastInc, sizeof C_minMaxAvg);
+  } else {
+    std::memcpy(C_volAvgLastInc, C_volAvg, sizeof C_volAvg);
+    std::memcpy(C_minMaxAvgLastInc, C_minMaxAvg, sizeof C_minMaxAvg);
+    double aim_diff[9];
+    for (int q = 0; q < 9; ++q) aim_diff[q] = (F_aim[q] - F_aim_last_inc[q]) / dt_old;
+    for (int q = 0; q < 9; ++q) F_aim_dot[q] = guess ? (stp.stress_mask[q] ? 0.0 : aim_diff[q]) : 0.0;
+    std::memcpy(F_aim_last_inc, F_aim, sizeof F_aim);
+    double step[9];
+    for (int q = 0; q < 9; ++q)
+      step[q] = (stp.deformation_type == SC_DEF_DOT_F) ? stp.deformation[q]
+                                                        : (stp.deformation[q] - F_aim_last_inc[q]) / t_remaining;
+    for (int q = 0; q < 9; ++q) F_aim_dot[q] += stp.deformation_mask[q] ? 0.0 : step[q];
+    double rotated[9];
+    h::rot2(stp.rot_q, F_aim_dot, rotated);
+    SCA_HIP(hipMemcpyAsync(small + SM_AV, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
+    launch_rate(guess ? 1 : 0, F_last_inc, X, dt_old, small + SM_AV, F_dot, N, st);
+    SCA_HIP(hipMemcpyAsync(F_last_inc, X, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    launch_mp_forward(mp, st);  // f_materialpoint_forward
+  }
+  for (int q = 0; q < 9; ++q) F_aim[q] = F_aim_last_inc[q] + F_aim_dot[q] * dt;
+  for (int q = 0; q < 9; ++q)
+    if (stp.stress_mask[q]) {
+      if (stp.stress_type == SC_STRESS_P)
+        P_aim[q] += (stp.stress[q] - P_aim[q]) / t_remaining * dt;
+      else
+        P_aim[q] += stp.stress[q] * dt;
+    }
+  double rotated[9];
+  h::rot2(stp.rot_q, F_aim, rotated);
+  SCA_HIP(hipMemcpyAsync(small + SM_AIM2, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
+  launch_fwd_field(F_last_inc, F_dot, dt, X, 9 * N, st);
+  std::vector<int> ops(9, 2);
+  std::vector<const double*> a(9);
+  for (int q = 0; q < 9; ++q) a[q] = X + q * N;
+  launch_partial_reduce<double>(ops.data(), N, 9, a.data(), nullptr, partial, st);
+  launch_final_reduce(partial, 9, red_blocks(), small + SM_FSUM, st);
+  launch_fwd_diff(small + SM_FSUM, wgt, small + SM_AIM2, small + SM_FDIFF, st);
+  launch_sub_plane_const(X, small + SM_FDIFF, N, st);
+  SCA_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < 9; ++q) stress_mask[q] = stp.stress_mask[q];
+  for (int q = 0; q < 4; ++q) rot_q[q] = stp.rot_q[q];
+  delta_t = dt;
+}
+
+sc_solution Rve::calculate_solution() {
+  masked_compliance_host(C_volAvg, rot_q, stress_mask, S);
+  if (desc.grid_num.update_gamma)
+    for (int q = 0; q < 81; ++q) C_ref[q] = C_minMaxAvg[q] / wgt;
+  const int reason = snes_ngmres();
+  tangent();
+  sc_solution out{};
+  out.converged = reason > 0;
+  out.stag_converged = 1;
+  out.required_iterations = total_iter;
+  out.terminally_ill = terminally_ill;
+  out.residual_evaluations = solve_evals;
+  out.err_div = err_div;
+  out.err_BC = err_BC;
+  terminally_ill = false;
+  for (int q = 0; q < 9; ++q)
+    if (stress_mask[q]) P_aim[q] = P_av[q];
+  return out;
+}
+
+int Rve::run_loadcase(const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations,
+                      int max_incs) {
+  const auto& gn = desc.grid_num;
+  bool cut_back = false;
+  const int sub_step_factor = 2;
+  int cut_back_level = 0, total_inc = 0;
+  double t = 0, dt = 1, dt_prev = 0, t_remaining = 0;
+  for (int l = 0; l < n_steps; ++l) {
+    const sc_loadstep& stp = steps[l];
+    const double t_0 = t;
+    bool guess = stp.estimate_rate && l > 0;
+    for (int inc = 0; inc < stp.N; ++inc) {
+      total_inc++;
+      dt_prev = dt;
+      if (std::fabs(stp.r - 1.0) < 1.e-9)
+        dt = stp.t / stp.N;
+      else
+        dt = stp.t * (std::pow(stp.r, inc) - std::pow(stp.r, inc + 1)) / (1.0 - std::pow(stp.r, stp.N));
+      dt *= std::pow((double)sub_step_factor, -(double)cut_back_level);
+      int step_fraction = 0, iters = 0, evals = 0;
+      while (step_fraction < std::pow(sub_step_factor, cut_back_level)) {
+        t_remaining = stp.t + t_0 - t;
+        t += dt;
+        step_fraction++;
+        forward(cut_back, guess, dt, dt_prev, t_remaining, stp);
+        const sc_solution sol = calculate_solution();
+        iters += sol.required_iterations;
+        evals += sol.residual_evaluations;
+        if (sol.converged && sol.stag_converged && !sol.terminally_ill) {
+          update_coords();
+          dt_prev = dt;
+          cut_back = false;
+          guess = true;
+        } else if (cut_back_level < gn.max_cut_back) {
+          cut_back = true;
+          step_fraction = (step_fraction - 1) * sub_step_factor;
+          cut_back_level++;
+          t = t - dt;
+          dt = dt / sub_step_factor;
+          stats.cutbacks++;
+        } else {
+          g_err = "no solution found for increment " + std::to_string(total_inc);
+          return 2;
+        }
+      }
+      cut_back_level = std::max(0, (int)cut_back - 1);  // damask_grid.cpp:153
+      stats.increments++;
+      if (inc_iterations && total_inc <= max_incs) inc_iterations[total_inc - 1] = iters;
+      if (inc_evaluations && total_inc <= max_incs) inc_evaluations[total_inc - 1] = evals;
+    }
+  }
+  return 0;
+}
+
+}  // namespace sca

@@ -1,0 +1,78 @@
+// Spectral-stage kernels (spectral.hip): FFT plans, Gamma/divergence,
+// deterministic reductions and streaming field operations.
+#pragma once
+#include <vector>
+
+#include "common.h"
+
+namespace sca {
+
+constexpr int MAX_RED_VEC = 96;
+
+// 3D FFT of `ncomp` real planes (N each) <-> half spectra (K complex each)
+template <typename T>
+struct FftPlan {
+  int cells[3] = {0, 0, 0};
+  int nr = 0;
+  int64_t N = 0, K = 0;
+  T* tw[3] = {nullptr, nullptr, nullptr};  // forward twiddles per axis
+  void init(const int cells[3]);
+  void release();
+  // real: ncomp planes of N; spec: ncomp planes of K complex (interleaved re, im)
+  void forward(const T* real, T* spec, int ncomp, hipStream_t st) const;
+  void backward(T* spec, T* real, int ncomp, double scale, hipStream_t st) const;  // spec is clobbered
+  void strided_passes(void* spec, int ncomp, int sign, hipStream_t st) const;
+};
+
+struct GammaHost {
+  int cells[3];
+  double size[3];
+  double rescale[3];
+  double C_ref[81];  // (l,n,m,o) at 27l+9n+3m+o (row-major)
+  int derivative;
+  int do_gamma;
+};
+int gamma_blocks(int64_t K);
+template <typename T>
+void launch_gamma_div(T* spec, int64_t K, const GammaHost& g, const double* d_aim, const double* d_xi_tab,
+                      double* div_partial, hipStream_t st);
+
+// base_update_coords: F_hat -> u_hat (planes 0..2) and the 9 sums of F;
+// then node / cell-centre coordinates from u (3 planes of N) and the sums
+void launch_coords_uhat(double* spec, int64_t K, const GammaHost& g, const double* d_xi_tab, double* fsum,
+                        hipStream_t st);
+void launch_coords_nodes(const double* u, const double* fsum, const int cells[3], const double size[3], double* x_n,
+                         hipStream_t st);
+void launch_coords_points(const double* u, const double* fsum, const int cells[3], const double size[3],
+                          double* x_p, hipStream_t st);
+
+int red_blocks();
+// ops[v] 0: dot(a_v, b_v); 1: sum (a_v - b_v)^2; 2: sum a_v.  partial: nvec x red_blocks()
+template <typename T>
+void launch_partial_reduce(const int* ops, int64_t n, int nvec, const T* const* a, const T* const* b,
+                           double* partial, hipStream_t st);
+// cells of C_minMaxAvg: first cell of largest / smallest |dPdF|^2 with norms
+// within TANGENT_TIE_REL of the extreme counted as tied.  n2: N doubles of
+// scratch, pv: 2*red_blocks() doubles, pi: 2*red_blocks() long longs.
+constexpr double TANGENT_TIE_REL = 1e-10;
+void tangent_minmax_select(const double* dPdF, int64_t N, double* n2, double* pv, long long* pi, int64_t& imax,
+                           int64_t& imin, hipStream_t st);
+void launch_gather81(const double* d, int64_t N, int64_t c, double* out, hipStream_t st);
+void launch_bc_aim(const double* sums, const double S[81], const double R[9], double wgt, int rotate, double* aim,
+                   hipStream_t st);
+void launch_final_reduce(const double* partial, int nvec, int nb, double* out, hipStream_t st);
+
+void launch_fwd_field(const double* last, const double* rate, double dt, double* out, int64_t n, hipStream_t st);
+void launch_sub_plane_const(double* f, const double* diff, int64_t N, hipStream_t st);
+void launch_fwd_diff(const double* sums, double wgt, const double* aim, double* diff, hipStream_t st);
+void launch_rate(int het, const double* f0, const double* f, double dt, const double* av, double* out, int64_t N,
+                 hipStream_t st);
+void launch_axpby(double a, const double* x, double b, const double* y0, double* y, int64_t n, hipStream_t st);
+void launch_lincomb(const double* XM, double c0, int l, const double* beta, const double* const* X, double* XA,
+                    int64_t n, hipStream_t st);
+void launch_soa_to_aos(const double* soa, double* aos, int64_t N, int ncomp, hipStream_t st);
+void launch_aos_to_soa(const double* aos, double* soa, int64_t N, int ncomp, hipStream_t st);
+void launch_f64_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
+void launch_f32_to_f64(const float* a, double* b, int64_t n, hipStream_t st);
+
+}  // namespace sca

@@ -29,6 +29,21 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
+def product():
+    """The HIP library.  A missing build is an error, never a skip."""
+    if not os.path.exists(PRODUCT_SO):
+        raise FileNotFoundError(f"{PRODUCT_SO} is missing: python -m solver_cpp_amd.build")
+    from solver_cpp_amd.capi import Library
+    return Library(PRODUCT_SO)
+
+
+@pytest.fixture(scope="session", params=["oracle", pytest.param("product", marks=pytest.mark.gpu)])
+def lib(request):
+    """Both implementations behind the same C ABI (the HIP one needs a GPU)."""
+    return request.getfixturevalue(request.param)
+
+
+@pytest.fixture(scope="session")
 def goldens():
     with open(os.path.join(GOLDEN, "reference_goldens.json")) as f:
         return json.load(f)

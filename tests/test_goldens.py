@@ -1,6 +1,10 @@
-"""Pins the CPU oracle against the reference's own golden vectors
-(src/grid/test/*.cpp), the original-DAMASK trace (output_damask_f.sh),
-closed forms and invariants.  CPU only."""
+"""Golden vectors of the reference's own tests (src/grid/test/*.cpp), the
+original-DAMASK trace (output_damask_f.sh), closed forms and invariants.
+
+Every test runs twice through the same C ABI: on the CPU oracle (which pins
+the oracle) and on the HIP library (marked gpu).  The oracle's run is the
+check that the restatement is right; the HIP run is parity with the
+reference's vectors."""
 import ctypes as C
 
 import numpy as np
@@ -14,96 +18,96 @@ IP = C.POINTER(C.c_int)
 Q2 = [0.7936696712125002, -0.28765777461664166, -0.3436487135089419, 0.4113964260949434]
 
 
-def test_xi_tables(oracle, goldens):
+def test_xi_tables(lib, goldens):
     g = goldens["spectral_init_2x1x1"]
     cells = np.array(g["cells"], dtype=np.int32)
     size = np.array(g["size"])
     n = 2 * 3 * (cells[0] // 2 + 1) * cells[1] * cells[2]
     xi1, xi2 = np.zeros(n), np.zeros(n)
-    oracle.sc_spectral_xi(ptr(cells, IP), ptr(size), 0, ptr(xi1), ptr(xi2))
+    lib.sc_spectral_xi(ptr(cells, IP), ptr(size), 0, ptr(xi1), ptr(xi2))
     np.testing.assert_allclose(xi1, cplx_interleaved(arr(g["xi1st"])), atol=1e-8, rtol=0)
     np.testing.assert_allclose(xi2, cplx_interleaved(arr(g["xi2nd"])), atol=1e-8, rtol=1e-14)
 
 
-def test_scaled_geom_size(oracle, goldens):
+def test_scaled_geom_size(lib, goldens):
     g = goldens["scaled_geom_size"]
     cells = np.array(g["cells"], dtype=np.int32)
     size = np.array(g["size"], dtype=np.float64)
     for dc, key in [(0, "none"), (1, "size_only")]:
         out = np.zeros(3)
-        oracle.sc_scaled_geom_size(ptr(cells, IP), ptr(size), dc, ptr(out))
+        lib.sc_scaled_geom_size(ptr(cells, IP), ptr(size), dc, ptr(out))
         np.testing.assert_allclose(out, g[key], rtol=1e-15)
     # size+grid: the golden (0.6, 0.8, 1) contradicts both mech_base.cpp:20-30
     # and DAMASK's Fortran (scale by the dimension whose size/cells is neither
     # the smallest nor the largest): size/cells = (1.5, 4, 5) -> dimension 2,
     # giving size/4*1.  The oracle follows the code.
     out = np.zeros(3)
-    oracle.sc_scaled_geom_size(ptr(cells, IP), ptr(size), 2, ptr(out))
+    lib.sc_scaled_geom_size(ptr(cells, IP), ptr(size), 2, ptr(out))
     np.testing.assert_allclose(out, [0.75, 1.0, 1.25], rtol=1e-15)
     cells2 = np.array([3, 8, 2], dtype=np.int32)  # size/cells = (1, 0.5, 2.5) -> dimension 0
-    oracle.sc_scaled_geom_size(ptr(cells2, IP), ptr(size), 2, ptr(out))
+    lib.sc_scaled_geom_size(ptr(cells2, IP), ptr(size), 2, ptr(out))
     np.testing.assert_allclose(out, size / 3.0 * 3, rtol=1e-15)
 
 
-def test_divergence_rms(oracle, goldens):
+def test_divergence_rms(lib, goldens):
     g = goldens["divergence_rms_4x1x1"]
     cells = np.array(g["cells"], dtype=np.int32)
     rms = np.zeros(1)
-    oracle.sc_divergence_rms(ptr(cells, IP), ptr(np.array(g["size"])), ptr(np.array(g["scaled_size"], dtype=float)),
+    lib.sc_divergence_rms(ptr(cells, IP), ptr(np.array(g["size"])), ptr(np.array(g["scaled_size"], dtype=float)),
                              ptr(cplx_interleaved(arr(g["xi1st"]))), ptr(arr(g["field"])), ptr(rms))
     # the reference asserts exact equality with its FFTW; 4 ulp covers a different FFT's rounding
     assert abs(rms[0] / g["expected_rms"] - 1.0) < 1e-15 * 4
 
 
-def test_masked_compliance(oracle, goldens):
+def test_masked_compliance(lib, goldens):
     g = goldens["masked_compliance"]
     mask = np.array(g["stress_mask_rowmajor"], dtype=np.int32).T.reshape(-1).copy()  # column-major
     S = np.zeros(81)
-    rc = oracle.sc_masked_compliance(ptr(arr(g["C"])), ptr(np.array(g["rot_q"])), ptr(mask, IP), ptr(S))
+    rc = lib.sc_masked_compliance(ptr(arr(g["C"])), ptr(np.array(g["rot_q"])), ptr(mask, IP), ptr(S))
     assert rc == 0
     exp = arr(g["expected"])
     np.testing.assert_allclose(S, exp, rtol=1e-12, atol=1e-24)
 
 
-def test_forward_field(oracle, goldens):
+def test_forward_field(lib, goldens):
     g = goldens["forward_field_2x1x1"]
     out = np.zeros(18)
-    oracle.sc_forward_field(2, g["delta_t"], ptr(arr(g["field_last_inc"])), ptr(arr(g["rate"])),
+    lib.sc_forward_field(2, g["delta_t"], ptr(arr(g["field_last_inc"])), ptr(arr(g["rate"])),
                             ptr(arr(g["aim"])), ptr(out))
     np.testing.assert_allclose(out, arr(g["expected"]), atol=1e-14)
 
 
-def test_calculate_rate(oracle, goldens):
+def test_calculate_rate(lib, goldens):
     g = goldens["calculate_rate_2x1x1"]
     out = np.zeros(18)
-    oracle.sc_calculate_rate(2, 1, ptr(arr(g["field_0"])), ptr(arr(g["field"])), g["dt"], ptr(arr(g["avRate"])),
+    lib.sc_calculate_rate(2, 1, ptr(arr(g["field_0"])), ptr(arr(g["field"])), g["dt"], ptr(arr(g["avRate"])),
                              ptr(out))
     np.testing.assert_allclose(out, arr(g["expected"]), rtol=1e-12, atol=1e-18)
 
 
-def test_gamma_convolution_dc(oracle, goldens):
+def test_gamma_convolution_dc(lib, goldens):
     g = goldens["gamma_convolution_2x1x1"]
     cells = np.array([2, 1, 1], dtype=np.int32)
     xi = np.zeros(2 * 3 * 2)
     field = arr(g["field"]).copy()
-    oracle.sc_gamma_convolution(ptr(cells, IP), ptr(xi), ptr(arr(g["C_ref"])), ptr(arr(g["field_aim"])), ptr(field))
+    lib.sc_gamma_convolution(ptr(cells, IP), ptr(xi), ptr(arr(g["C_ref"])), ptr(arr(g["field_aim"])), ptr(field))
     np.testing.assert_allclose(field, arr(g["expected"]), atol=1e-18, rtol=1e-13)
 
 
-def test_response_reductions(oracle, goldens):
+def test_response_reductions(lib, goldens):
     g = goldens["constitutive_response_2x1x1"]
     P_av, Cv, Cm = np.zeros(9), np.zeros(81), np.zeros(81)
-    oracle.sc_response_reductions(2, ptr(arr(g["P"])), ptr(arr(g["dPdF"])), None, ptr(P_av), ptr(Cv), ptr(Cm))
+    lib.sc_response_reductions(2, ptr(arr(g["P"])), ptr(arr(g["dPdF"])), None, ptr(P_av), ptr(Cv), ptr(Cm))
     np.testing.assert_allclose(P_av, arr(g["P_av"]), rtol=1e-14, atol=1e-6)
     np.testing.assert_allclose(Cv, arr(g["C_volAvg"]), rtol=1e-14, atol=1e-4)
     np.testing.assert_allclose(Cm, arr(g["C_minMaxAvg"]), rtol=1e-14, atol=1e-4)
 
 
-def test_form_residual_bc_update(oracle, goldens):
+def test_form_residual_bc_update(lib, goldens):
     """S:P_av and the F_aim update of form_residual (mech_solver_basic.cpp:322-324)."""
     g = goldens["form_residual_bc"]
     d = np.zeros(9)
-    oracle.f_math_mul3333xx33(ptr(arr(g["S"])), ptr(arr(g["P_av"])), ptr(d))
+    lib.f_math_mul3333xx33(ptr(arr(g["S"])), ptr(arr(g["P_av"])), ptr(d))
     np.testing.assert_allclose(d, arr(g["expected_rotated"]), rtol=1e-14, atol=1e-20)
     F_aim = np.array(g["F_aim"]).T.reshape(-1)
     np.testing.assert_allclose(F_aim - d, arr(g["expected_F_aim"]), rtol=1e-15)
@@ -117,9 +121,9 @@ def _two_point_rve(lib, **kw):
     return Rve(lib, (2, 1, 1), (2e-5, 1e-5, 1e-5), np.array([0, 1]), ori, **kw)
 
 
-def test_basic_forward(oracle, goldens):
+def test_basic_forward(lib, goldens):
     g = goldens["basic_forward_2x1x1"]
-    R = _two_point_rve(oracle)
+    R = _two_point_rve(lib)
     R.set_field("F", np.tile(np.eye(3).reshape(-1), 2))
     load = tension_x()
     R.forward(False, False, g["delta_t"], g["delta_t_old"], g["t_remaining"], load)
@@ -127,28 +131,31 @@ def test_basic_forward(oracle, goldens):
     np.testing.assert_allclose(np.array(R.scalars().F_aim), arr(g["expected_aim"]), rtol=1e-15)
 
 
-def test_update_coords_identity(oracle, goldens):
+def test_update_coords_identity(lib, goldens):
     g = goldens["update_coords_2x1x1"]
-    R = _two_point_rve(oracle)
+    R = _two_point_rve(lib)
     R.init()  # F = I, base_update_coords
     np.testing.assert_allclose(R.field("x_n"), arr(g["x_n"]), atol=1e-20, rtol=1e-14)
     np.testing.assert_allclose(R.field("x_p"), arr(g["x_p"]), atol=1e-20, rtol=1e-14)
 
 
-def test_constitutive_response_loadcase(oracle, goldens):
+def test_constitutive_response_loadcase(lib, goldens):
     """The reference's SpectralTestConstitutiveResponse values are the state
     of a 2-cell run of tensionX (dot_F_xx = 1e-3/s) after 10 increments of
     1 s with the example material.yaml orientations 1 and 2; the oracle's full
     path (load-case loop, NGMRES, material point) reproduces them."""
     g = goldens["constitutive_response_2x1x1"]
-    R = _two_point_rve(oracle)
+    R = _two_point_rve(lib)
     R.init()
     its, evs = R.loop_over_loadcase([tension_x(t=10, N=10)])
     assert (its >= 1).all()
-    np.testing.assert_allclose(R.field("F"), arr(g["F"]), rtol=1e-12, atol=1e-15)
+    # the oracle lands within 1e-12 of the reference's values; the GPU form
+    # (other summation orders in every reduction) is held to the 1e-10 bar
+    tol = 1e-12 if lib.backend == "cpu-oracle" else 1e-10
+    np.testing.assert_allclose(R.field("F"), arr(g["F"]), rtol=tol, atol=1e-15)
     P_ref = arr(g["P"])
     P = R.field("P")
-    assert np.abs(P - P_ref).max() < 1e-11 * np.abs(P_ref).max()
+    assert np.abs(P - P_ref).max() < 10 * tol * np.abs(P_ref).max()
     s = R.scalars()
     np.testing.assert_allclose(np.array(s.P_av), arr(g["P_av"]), rtol=1e-10, atol=1e-3)
     Cv, Cm = np.array(s.C_volAvg), np.array(s.C_minMaxAvg)
@@ -157,7 +164,7 @@ def test_constitutive_response_loadcase(oracle, goldens):
     assert np.abs(Cm - arr(g["C_minMaxAvg"])).max() < 1e-9 * scale
 
 
-def test_ngmres_replays_fortran_trace(oracle, trace):
+def test_ngmres_replays_fortran_trace(lib, trace):
     """Each of the 10 increments of the original-DAMASK run: starting from the
     recorded first F, feeding the recorded residuals, the solver must request
     exactly the recorded F at every evaluation (Richardson candidates,
@@ -169,7 +176,7 @@ def test_ngmres_replays_fortran_trace(oracle, trace):
         x0 = np.ascontiguousarray(F[idx[0]])
         rr = np.ascontiguousarray(r[idx])
         xe = np.zeros_like(rr)
-        rc = oracle.sc_ngmres_replay(243, 30, ptr(x0), len(idx), ptr(rr), ptr(xe))
+        rc = lib.sc_ngmres_replay(243, 30, ptr(x0), len(idx), ptr(rr), ptr(xe))
         assert rc == 0
         np.testing.assert_allclose(xe, F[idx], atol=1e-14, rtol=0)
     # iterations per increment (SURVEY counts total_iter values 0..k)
@@ -178,14 +185,14 @@ def test_ngmres_replays_fortran_trace(oracle, trace):
 
 
 @pytest.mark.parametrize("cells", [(4, 3, 2), (5, 2, 3), (8, 6, 4), (6, 10, 5)])
-def test_fft_matches_dft(oracle, cells):
+def test_fft_matches_dft(lib, cells):
     rng = np.random.default_rng(1)
     Nx, Ny, Nz = cells
     f = rng.standard_normal((Nz, Ny, Nx))  # x fastest in memory
     c = np.array(cells, dtype=np.int32)
     nr = Nx // 2 + 1
     out = np.zeros(2 * nr * Ny * Nz)
-    oracle.sc_fft_r2c(ptr(c, IP), 1, ptr(f.reshape(-1).copy()), ptr(out))
+    lib.sc_fft_r2c(ptr(c, IP), 1, ptr(f.reshape(-1).copy()), ptr(out))
     got = (out[0::2] + 1j * out[1::2]).reshape(Ny, Nz, nr)  # (nr, Nz, Ny) i fastest
     # brute-force DFT from the definition, e^{-2 pi i k.x/N}
     x = np.arange(Nx)[None, None, :]
@@ -199,7 +206,7 @@ def test_fft_matches_dft(oracle, cells):
                 exp[j, k, i] = np.sum(f * ph)
     np.testing.assert_allclose(got, exp, atol=1e-11 * np.abs(exp).max())
     back = np.zeros(f.size)
-    oracle.sc_fft_c2r(ptr(c, IP), 1, ptr(out), 1.0 / f.size, ptr(back))
+    lib.sc_fft_c2r(ptr(c, IP), 1, ptr(out), 1.0 / f.size, ptr(back))
     np.testing.assert_allclose(back.reshape(f.shape), f, atol=1e-13)
 
 
@@ -207,7 +214,7 @@ def _rot_C(C66_iso, q):
     return C66_iso
 
 
-def test_gamma_reproduces_compatible_field(oracle):
+def test_gamma_reproduces_compatible_field(lib):
     """Gamma(C_ref : grad u) = grad u for a single-mode displacement u and a
     homogeneous reference stiffness (C_ref = C / wgt): the projection
     reproduces compatible fields, which fixes the operator's index placement,
@@ -218,9 +225,9 @@ def test_gamma_reproduces_compatible_field(oracle):
     nr = cells[0] // 2 + 1
     K = nr * cells[1] * cells[2]
     xi1, xi2 = np.zeros(6 * K), np.zeros(6 * K)
-    oracle.sc_spectral_xi(ptr(cells, IP), ptr(size), 0, ptr(xi1), ptr(xi2))
+    lib.sc_spectral_xi(ptr(cells, IP), ptr(size), 0, ptr(xi1), ptr(xi2))
     # cubic stiffness in a rotated frame (anisotropic, no symmetry shortcuts)
-    m, _, _ = oracle.defaults()
+    m, _, _ = lib.defaults()
     C66 = np.zeros((6, 6))
     C66[:3, :3] = m.C_12
     np.fill_diagonal(C66[:3, :3], m.C_11)
@@ -235,7 +242,7 @@ def test_gamma_reproduces_compatible_field(oracle):
     Cf = Cl.reshape(-1, order="F").copy()
     Crot = np.zeros(81)
     act = C.c_int(0)
-    oracle.f_rotate_tensor4(ptr(np.array(Q2)), ptr(Cf), ptr(Crot), C.byref(act))
+    lib.f_rotate_tensor4(ptr(np.array(Q2)), ptr(Cf), ptr(Crot), C.byref(act))
     Cc = Crot.reshape(3, 3, 3, 3, order="F")
     wgt = 1.0 / N
     C_ref = (Cc / wgt).reshape(-1, order="F").copy()
@@ -250,19 +257,19 @@ def test_gamma_reproduces_compatible_field(oracle):
     grad = np.einsum("l,m,zyx->zyxlm", a, g, np.cos(phase))  # (z,y,x,l,m)
     P = np.einsum("lmno,zyxno->zyxlm", Cc, grad)
     field = np.ascontiguousarray(P.transpose(0, 1, 2, 4, 3)).reshape(-1)  # (3,3) column-major per cell
-    oracle.sc_gamma_convolution(ptr(cells, IP), ptr(xi1), ptr(C_ref), ptr(np.zeros(9)), ptr(field))
+    lib.sc_gamma_convolution(ptr(cells, IP), ptr(xi1), ptr(C_ref), ptr(np.zeros(9)), ptr(field))
     got = field.reshape(Nz := cells[2], cells[1], cells[0], 3, 3).transpose(0, 1, 2, 4, 3)
     np.testing.assert_allclose(got, grad, atol=1e-12 * np.abs(grad).max())
 
 
-def test_gamma_annihilates_equilibrated_stress(oracle):
+def test_gamma_annihilates_equilibrated_stress(lib):
     """A stress mode with P_hat . xi = 0 carries no residual (only DC remains)."""
     cells = np.array([6, 4, 4], dtype=np.int32)
     size = np.array([6e-5, 4e-5, 4e-5])
     N = int(np.prod(cells))
     K = (cells[0] // 2 + 1) * cells[1] * cells[2]
     xi1, xi2 = np.zeros(6 * K), np.zeros(6 * K)
-    oracle.sc_spectral_xi(ptr(cells, IP), ptr(size), 0, ptr(xi1), ptr(xi2))
+    lib.sc_spectral_xi(ptr(cells, IP), ptr(size), 0, ptr(xi1), ptr(xi2))
     C_ref = (np.eye(9) * 1e11 * N).reshape(-1).copy()  # any invertible reference stiffness
     kv = np.array([1, 1, 0])
     cdir = np.array([1.0, -1.0, 0.3])  # c . k = 0 (in physical units: sizes differ)
@@ -277,7 +284,7 @@ def test_gamma_annihilates_equilibrated_stress(oracle):
     field = np.ascontiguousarray(P.transpose(0, 1, 2, 4, 3)).reshape(-1)
     aim = np.zeros(9)
     aim[0] = 1e-3
-    oracle.sc_gamma_convolution(ptr(cells, IP), ptr(xi1), ptr(C_ref), ptr(aim), ptr(field))
+    lib.sc_gamma_convolution(ptr(cells, IP), ptr(xi1), ptr(C_ref), ptr(aim), ptr(field))
     got = field.reshape(-1, 9)
     exp = np.tile(aim, (N, 1))
     np.testing.assert_allclose(got, exp, atol=1e-15)
@@ -321,12 +328,12 @@ def _qu2om(lib, q):
     return om
 
 
-def test_mp_elastic_limit_and_invariants(oracle):
+def test_mp_elastic_limit_and_invariants(lib):
     """Small strain, identity orientation: P -> C:eps; det Fp = 1; S symmetric."""
-    m, _, _ = oracle.defaults()
+    m, _, _ = lib.defaults()
     eps = np.array([[2e-6, 1e-6, 0], [1e-6, -1e-6, 3e-7], [0, 3e-7, 5e-7]])
     F = (np.eye(3) + eps).T.reshape(1, 9).copy()
-    out = _run_mp(oracle, F, np.eye(3).reshape(1, 9).copy(), 1.0)
+    out = _run_mp(lib, F, np.eye(3).reshape(1, 9).copy(), 1.0)
     assert out["broken"][0] == 0
     P = out["P"][0].reshape(3, 3, order="F")
     tr = np.trace(eps)
@@ -343,13 +350,13 @@ def test_mp_elastic_limit_and_invariants(oracle):
     assert abs(np.linalg.det(Fp) - 1.0) < 1e-14
 
 
-def test_mp_tangent_matches_finite_differences(oracle):
+def test_mp_tangent_matches_finite_differences(lib):
     """dPdF (phase_mechanical_dPdF) against central differences of P(F) at a
     plastically active state (rotated grain, 0.5 % stretch in one step)."""
-    Fp0 = _qu2om(oracle, Q2).reshape(1, 9)
+    Fp0 = _qu2om(lib, Q2).reshape(1, 9)
     F0 = np.diag([1.005, 0.998, 0.9975]) + np.array([[0, 1e-4, 0], [0, 0, 2e-4], [0, 0, 0]])
     Fc = F0.T.reshape(1, 9).copy()
-    ref = _run_mp(oracle, Fc, Fp0, 5.0, dPdF=True, tight=True)
+    ref = _run_mp(lib, Fc, Fp0, 5.0, dPdF=True, tight=True)
     assert ref["broken"][0] == 0
     assert np.abs(ref["Lp"]).max() > 1e-6  # plastic flow is active
     h = 1e-6
@@ -360,7 +367,7 @@ def test_mp_tangent_matches_finite_differences(oracle):
             Fq[0, q] += sgn * h
             Fs.append(Fq[0])
     Fs = np.array(Fs)
-    res = _run_mp(oracle, Fs, np.tile(Fp0, (18, 1)), 5.0, Lp=np.tile(ref["Lp"], (18, 1)), tight=True)
+    res = _run_mp(lib, Fs, np.tile(Fp0, (18, 1)), 5.0, Lp=np.tile(ref["Lp"], (18, 1)), tight=True)
     fd = np.zeros((9, 9))  # fd[P component, F component]
     for q in range(9):
         fd[:, q] = (res["P"][2 * q] - res["P"][2 * q + 1]) / (2 * h)

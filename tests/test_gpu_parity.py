@@ -1,0 +1,251 @@
+"""Parity of the HIP library with the CPU oracle on the same seeded inputs.
+
+Integer results (iteration counts, broken flags) must be identical; FP64
+fields agree within 1e-10 relative to the field's magnitude, the bar the
+plain GPU form has to meet against the reference solver.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from solver_cpp_amd.capi import ptr
+from solver_cpp_amd.geometry import polycrystal, uniform_orientations
+from solver_cpp_amd.rve import Rve, tension_x
+
+pytestmark = pytest.mark.gpu
+IP = C.POINTER(C.c_int)
+REL = 1e-10  # per-point tolerance of the plain FP64 form
+
+
+def close(a, b, rel=REL, what=""):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max() / scale
+    assert err <= rel, f"{what}: max |diff| / max |ref| = {err:.3e} > {rel:.1e}"
+
+
+def test_backend(product):
+    assert product.backend == "hip-gfx950"
+
+
+# ---- spectral stages ---------------------------------------------------------------
+FFT_CELLS = [(2, 1, 1), (4, 3, 5), (8, 8, 8), (6, 10, 7), (16, 12, 9), (32, 32, 32), (20, 14, 22)]
+
+
+@pytest.mark.parametrize("cells", FFT_CELLS)
+def test_fft_r2c_c2r(product, oracle, cells):
+    rng = np.random.default_rng(sum(cells))
+    c = np.array(cells, dtype=np.int32)
+    N = int(np.prod(cells))
+    K = (cells[0] // 2 + 1) * cells[1] * cells[2]
+    howmany = 9
+    f = rng.standard_normal(howmany * N)
+    a, b = np.zeros(2 * howmany * K), np.zeros(2 * howmany * K)
+    product.check(product.sc_fft_r2c(ptr(c, IP), howmany, ptr(f), ptr(a)), "fft_r2c")
+    oracle.sc_fft_r2c(ptr(c, IP), howmany, ptr(f), ptr(b))
+    close(a, b, 1e-13, "r2c")
+    back = np.zeros_like(f)
+    product.check(product.sc_fft_c2r(ptr(c, IP), howmany, ptr(a), 1.0 / N, ptr(back)), "fft_c2r")
+    close(back, f, 1e-13, "c2r(r2c(f)) / N")
+
+
+def _xi(lib, cells, size, derivative=0):
+    c = np.array(cells, dtype=np.int32)
+    K = (cells[0] // 2 + 1) * cells[1] * cells[2]
+    x1, x2 = np.zeros(6 * K), np.zeros(6 * K)
+    lib.sc_spectral_xi(ptr(c, IP), ptr(np.array(size, dtype=float)), derivative, ptr(x1), ptr(x2))
+    return x1, x2
+
+
+@pytest.mark.parametrize("derivative", [0, 1, 2])
+def test_xi_tables(product, oracle, derivative):
+    cells, size = (6, 5, 4), (6e-5, 5e-5, 4e-5)
+    for a, b in zip(_xi(product, cells, size, derivative), _xi(oracle, cells, size, derivative)):
+        close(a, b, 1e-15, "xi")
+
+
+@pytest.mark.parametrize("cells", [(4, 4, 4), (6, 5, 8), (16, 16, 16)])
+def test_divergence_and_gamma(product, oracle, cells):
+    rng = np.random.default_rng(3)
+    c = np.array(cells, dtype=np.int32)
+    size = np.array(cells, dtype=float) * 1e-6
+    N = int(np.prod(cells))
+    xi1, _ = _xi(oracle, cells, size)
+    field = rng.standard_normal(9 * N) * 1e7
+    scaled = np.zeros(3)
+    oracle.sc_scaled_geom_size(ptr(c, IP), ptr(size), 2, ptr(scaled))
+    r1, r2 = np.zeros(1), np.zeros(1)
+    product.check(product.sc_divergence_rms(ptr(c, IP), ptr(size), ptr(scaled), ptr(xi1), ptr(field), ptr(r1)), "div")
+    oracle.sc_divergence_rms(ptr(c, IP), ptr(size), ptr(scaled), ptr(xi1), ptr(field), ptr(r2))
+    close(r1, r2, 1e-13, "divergence rms")
+    A = rng.standard_normal((9, 9))
+    C_ref = ((A @ A.T + 9 * np.eye(9)) * 1e10 * N).reshape(-1).copy()
+    aim = rng.standard_normal(9) * 1e-3
+    g1, g2 = field.copy(), field.copy()
+    product.check(product.sc_gamma_convolution(ptr(c, IP), ptr(xi1), ptr(C_ref), ptr(aim), ptr(g1)), "gamma")
+    oracle.sc_gamma_convolution(ptr(c, IP), ptr(xi1), ptr(C_ref), ptr(aim), ptr(g2))
+    close(g1, g2, 1e-12, "Gamma * P")
+
+
+def test_field_updates(product, oracle):
+    rng = np.random.default_rng(5)
+    n = 3000
+    last, rate, f0 = (rng.standard_normal(9 * n) for _ in range(3))
+    aim = rng.standard_normal(9)
+    o1, o2 = np.zeros(9 * n), np.zeros(9 * n)
+    product.check(product.sc_forward_field(n, 0.3, ptr(last), ptr(rate), ptr(aim), ptr(o1)), "forward_field")
+    oracle.sc_forward_field(n, 0.3, ptr(last), ptr(rate), ptr(aim), ptr(o2))
+    close(o1, o2, 1e-14, "forward_field")
+    av = rng.standard_normal(9)
+    for het in (0, 1):
+        product.check(product.sc_calculate_rate(n, het, ptr(f0), ptr(last), 0.7, ptr(av), ptr(o1)), "rate")
+        oracle.sc_calculate_rate(n, het, ptr(f0), ptr(last), 0.7, ptr(av), ptr(o2))
+        close(o1, o2, 1e-15, "calculate_rate")
+
+
+def test_response_reductions(product, oracle):
+    rng = np.random.default_rng(6)
+    n = 5000
+    P = rng.standard_normal(9 * n) * 1e8
+    d = rng.standard_normal(81 * n) * 1e10
+    q = uniform_orientations(1, 4)[0]
+    outs = []
+    for lib in (product, oracle):
+        pa, cv, cm = np.zeros(9), np.zeros(81), np.zeros(81)
+        lib.check(lib.sc_response_reductions(n, ptr(P), ptr(d), ptr(q), ptr(pa), ptr(cv), ptr(cm)), "reductions")
+        outs.append((pa, cv, cm))
+    for a, b in zip(*outs):
+        close(a, b, 1e-12, "reductions")
+
+
+# ---- material point ------------------------------------------------------------------
+def _mp_batch(lib, F, Fp0, dt, Lp=None, xi0=None):
+    n = F.shape[0]
+    m, p, _ = lib.defaults()
+    I9 = np.tile(np.eye(3).reshape(-1), (n, 1))
+    Z9 = np.zeros((n, 9))
+    xi0 = np.full((n, 12), m.xi_0_sl) if xi0 is None else xi0
+    g0 = np.zeros((n, 12))
+    Lp = np.zeros((n, 9)) if Lp is None else Lp.copy()
+    out = {k: np.zeros((n, 9)) for k in ["P", "Fp", "S", "Fe"]}
+    xi, ga = np.zeros((n, 12)), np.zeros((n, 12))
+    d = np.zeros((n, 81))
+    br = np.zeros(n, dtype=np.int32)
+    it = np.zeros(2 * n, dtype=np.int32)
+    rc = lib.sc_mp_update(n, dt, C.byref(m), C.byref(p), ptr(F), ptr(I9), ptr(Fp0), ptr(Z9), ptr(Z9), ptr(xi0),
+                          ptr(g0), ptr(Lp), ptr(out["P"]), ptr(out["Fp"]), ptr(out["S"]), ptr(out["Fe"]), ptr(xi),
+                          ptr(ga), ptr(d), ptr(br, IP), ptr(it, IP))
+    lib.check(rc, "sc_mp_update")
+    out.update(Lp=Lp, xi=xi, gamma=ga, dPdF=d, broken=br, iterations=it.reshape(n, 2))
+    return out
+
+
+def _random_batch(oracle, strain, n=4096):
+    rng = np.random.default_rng(int(strain * 1e5))
+    q = uniform_orientations(n, 11)
+    Fp0 = np.zeros((n, 9))
+    for c in range(n):
+        oracle.sc_qu2om(ptr(q[c]), ptr(Fp0[c]))
+    F = np.tile(np.eye(3).reshape(-1), (n, 1)) + strain * rng.uniform(-1, 1, (n, 9))
+    return np.ascontiguousarray(F), Fp0
+
+
+@pytest.mark.parametrize("strain", [5e-4, 1e-3, 2e-3])
+def test_mp_update(product, oracle, strain):
+    """Random grains and random deformation increments over dt = 0.5 s, from
+    the load case's own step (5e-4) to four times it (fully plastic, ~24
+    Newton iterations per point): identical iteration counts per point and
+    every output within 1e-10."""
+    F, Fp0 = _random_batch(oracle, strain)
+    a = _mp_batch(product, F, Fp0, 0.5)
+    b = _mp_batch(oracle, F, Fp0, 0.5)
+    np.testing.assert_array_equal(a["broken"], b["broken"])
+    np.testing.assert_array_equal(a["iterations"], b["iterations"])
+    for k in ["P", "Fp", "Lp", "S", "Fe", "xi", "gamma", "dPdF"]:
+        close(a[k], b[k], REL, k)
+    # warm start from the returned Lp (the Newton guess of the next evaluation)
+    a2 = _mp_batch(product, F, Fp0, 0.5, Lp=b["Lp"])
+    b2 = _mp_batch(oracle, F, Fp0, 0.5, Lp=b["Lp"])
+    np.testing.assert_array_equal(a2["iterations"], b2["iterations"])
+    close(a2["P"], b2["P"], REL, "P (warm start)")
+
+
+def test_mp_update_substepping(product, oracle):
+    """Steps of 2 % strain in 0.5 s force cutbacks and substeps.  The
+    integration's stopping tests (Newton, state fixed point) then sit close
+    to their thresholds for some points, so rounding decides between two
+    valid results: two CPU builds of the oracle (with and without fused
+    multiply-add) already disagree on 201 of these 4096 points.  Required:
+    the same broken flags, at most 10 % of points with other iteration
+    counts, and the rest within 1e-10."""
+    F, Fp0 = _random_batch(oracle, 2e-2)
+    a = _mp_batch(product, F, Fp0, 0.5)
+    b = _mp_batch(oracle, F, Fp0, 0.5)
+    np.testing.assert_array_equal(a["broken"], b["broken"])
+    same = (a["iterations"] == b["iterations"]).all(axis=1)
+    assert same.mean() >= 0.9, same.mean()
+    assert (b["iterations"][:, 1] > 12).any()  # substepping happened
+    for k in ["P", "Fp", "Lp", "xi", "gamma"]:
+        close(a[k][same], b[k][same], REL, k)
+
+
+# ---- outer solver ----------------------------------------------------------------------
+def test_ngmres_replays_fortran_trace(product, trace):
+    F, r, inc = trace["F"], trace["r"], trace["increment"]
+    for i in range(1, 11):
+        idx = np.where(inc == i)[0]
+        x0 = np.ascontiguousarray(F[idx[0]])
+        rr = np.ascontiguousarray(r[idx])
+        xe = np.zeros_like(rr)
+        product.check(product.sc_ngmres_replay(243, 30, ptr(x0), len(idx), ptr(rr), ptr(xe)), "replay")
+        np.testing.assert_allclose(xe, F[idx], atol=1e-14, rtol=0)
+
+
+# ---- full solver ------------------------------------------------------------------------
+def _pair(product, oracle, cells, grains, seed=1, **kw):
+    mat, ori, size = polycrystal(cells, grains, seed=seed)
+    return Rve(product, cells, size, mat, ori, **kw), Rve(oracle, cells, size, mat, ori, **kw)
+
+
+def test_init_and_single_residual(product, oracle):
+    a, b = _pair(product, oracle, (8, 6, 4), 5)
+    a.init()
+    b.init()
+    sa, sb = a.scalars(), b.scalars()
+    close(sa.C_minMaxAvg, sb.C_minMaxAvg, 1e-13, "C_minMaxAvg")
+    close(sa.C_volAvg, sb.C_volAvg, 1e-13, "C_volAvg")
+    close(a.field("x_p"), b.field("x_p"), 1e-14, "x_p")
+    close(a.field("x_n"), b.field("x_n"), 1e-14, "x_n")
+    load = tension_x(t=10.0, N=20)
+    for R in (a, b):
+        R.forward(False, False, 0.5, 1.0, 10.0, load)
+    close(a.field("F"), b.field("F"), 1e-13, "forward F")
+    # a solution then one more residual at a perturbed F
+    sol_a, sol_b = a.calculate_solution(), b.calculate_solution()
+    assert sol_a.required_iterations == sol_b.required_iterations
+    assert sol_a.residual_evaluations == sol_b.residual_evaluations
+    F = b.field("F") + 1e-5 * np.random.default_rng(2).standard_normal(9 * b.N)
+    ra, rb = a.form_residual(F), b.form_residual(F)
+    close(ra, rb, REL, "residual")
+    close(a.field("P"), b.field("P"), REL, "P")
+
+
+def test_loadcase_16cubed_20grains(product, oracle):
+    """Plain FP64 on the CPU-runnable configuration: 16^3 grid, 20 Voronoi
+    grains, uniaxial tension dot_F_xx = 1e-3/s to 1 % strain in 20 increments.
+    Per-point P and F within 1e-10, identical iteration counts per increment."""
+    a, b = _pair(product, oracle, (16, 16, 16), 20)
+    a.init()
+    b.init()
+    load = [tension_x(t=10.0, N=20)]
+    ia, ea = a.loop_over_loadcase(load)
+    ib, eb = b.loop_over_loadcase(load)
+    np.testing.assert_array_equal(ia, ib)
+    np.testing.assert_array_equal(ea, eb)
+    close(a.field("F"), b.field("F"), REL, "F")
+    close(a.field("P"), b.field("P"), REL, "P")
+    close(a.field("Fp"), b.field("Fp"), REL, "Fp")
+    close(a.field("xi"), b.field("xi"), REL, "xi")
+    close(a.scalars().P_av, b.scalars().P_av, REL, "P_av")
+    close(a.field("x_p"), b.field("x_p"), 1e-12, "x_p")
