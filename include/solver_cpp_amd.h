@@ -214,6 +214,28 @@ int sc_rve_get_scalars(sc_rve* rve, sc_solver_scalars* out);
 int sc_rve_get_stats(sc_rve* rve, sc_stats* out);
 int sc_rve_reset_stats(sc_rve* rve);
 
+/* Kernel timing (no counterpart in the reference; for measurement).  With
+ * profiling enabled every launch of the classes below is bracketed by HIP
+ * events on the context's stream, and the material-point kernels' Newton and
+ * state iterations are summed (then also reported in sc_stats). */
+enum {
+  SC_KT_RESPONSE = 0,  /* material-point update (stress integration)       */
+  SC_KT_TANGENT = 1,   /* consistent tangent dPdF                           */
+  SC_KT_FFT_FWD = 2,   /* forward 3D FFT of the 9 stress planes             */
+  SC_KT_FFT_BWD = 3,   /* inverse 3D FFT of the 9 residual planes           */
+  SC_KT_GAMMA = 4,     /* fused Gamma convolution + divergence              */
+  SC_KT_REDUCE = 5,    /* reductions and N-GMRES vector updates             */
+  SC_KT_OTHER = 6,
+  SC_KT_COUNT = 7
+};
+typedef struct sc_kernel_times {
+  int64_t launches[SC_KT_COUNT];
+  double  ms[SC_KT_COUNT];        /* summed event time */
+  int64_t points[SC_KT_COUNT];    /* summed cells (or wavevectors) processed */
+} sc_kernel_times;
+int sc_rve_profile(sc_rve* rve, int enable);
+int sc_rve_get_kernel_times(sc_rve* rve, sc_kernel_times* out);
+
 /* ------------------------------------------------------------------------- */
 /* Spectral building blocks (parity tests against the reference's goldens).  */
 /* All arrays are host arrays; the product runs them on the device.          */

@@ -90,6 +90,13 @@ class SolverScalars(C.Structure):
     ]
 
 
+KT_CLASSES = ["response", "tangent", "fft_fwd", "fft_bwd", "gamma", "reduce", "other"]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("launches", C.c_int64 * 7), ("ms", C.c_double * 7), ("points", C.c_int64 * 7)]
+
+
 SC_FIELD = dict(F=0, P=1, Fp=2, Lp=3, S=4, Fe=5, F_last_inc=6, xi=7, gamma=8, x_p=9, x_n=10)
 SC_DEF = dict(F=0, dot_F=1, L=2)
 SC_STRESS = dict(P=0, dot_P=1)
@@ -138,6 +145,12 @@ _SIGNATURES = {
     "f_rotate_tensor4": (None, [_dp, _dp, _dp, _ip]),
 }
 
+# Measurement hooks of the HIP library (bound when present).
+PROFILE_SIGNATURES = {
+    "sc_rve_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "sc_rve_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
+}
+
 # Symbols only the product exports (drop-in boundary of the Fortran core).
 BOUNDARY_SIGNATURES = {
     "f_materialpoint_initBase": (None, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
@@ -178,6 +191,12 @@ class Library:
             fn.restype = res
             fn.argtypes = args
             setattr(self, name, fn)
+        for name, (res, args) in PROFILE_SIGNATURES.items():
+            if hasattr(self.lib, name):
+                fn = getattr(self.lib, name)
+                fn.restype = res
+                fn.argtypes = args
+                setattr(self, name, fn)
 
     def check(self, rc: int, what: str = ""):
         if rc != 0:

@@ -390,6 +390,22 @@ int sc_rve_reset_stats(sc_rve* R) {
   return 0;
 }
 
+int sc_rve_profile(sc_rve* R, int enable) {
+  return guarded([&] {
+    R->r->kt.flush();
+    R->r->kt.on = enable != 0;
+    if (enable) R->r->kt.reset();
+    return 0;
+  });
+}
+int sc_rve_get_kernel_times(sc_rve* R, sc_kernel_times* out) {
+  return guarded([&] {
+    R->r->kt.flush();
+    *out = R->r->kt.acc;
+    return 0;
+  });
+}
+
 // ---- outer solver replay -------------------------------------------------------
 int sc_ngmres_replay(int64_t n, int m, const double* x0, int n_evals, const double* r_rec, double* x_eval) {
   return guarded([&] {

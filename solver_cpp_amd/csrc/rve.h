@@ -28,13 +28,37 @@ enum {
   SM_AIM2 = 140,  // 9: forward_field aim
   SM_FSUM = 150,  // 9: forward_field sums
   SM_FDIFF = 160, // 9: forward_field correction
+  SM_ITERS = 180, // 2: Newton / state iterations of the last response (profiling)
   SM_G81 = 200    // 162: gathered dPdF of argmax / argmin
+};
+
+// HIP-event timing of kernel classes (sc_kernel_times), off by default
+class KTimer {
+ public:
+  bool on = false;
+  sc_kernel_times acc{};
+  void begin(int cls, hipStream_t st);
+  void end(int cls, int64_t points, hipStream_t st);
+  void flush();  // waits for the recorded events and accumulates them
+  void reset() { acc = sc_kernel_times{}; }
+  ~KTimer();
+
+ private:
+  struct Rec {
+    int cls;
+    int64_t pts;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t open_a = nullptr;
+  hipEvent_t get();
 };
 
 // PETSc SNESNGMRES (default options, see solver.cpp) on device vectors
 class Ngmres {
  public:
-  Ngmres(int64_t n, hipStream_t st);
+  Ngmres(int64_t n, hipStream_t st, KTimer* kt = nullptr);
   ~Ngmres();
   // residual(x, f): f = F(x); conv(k): 1 converged, -1 diverged, 0 iterating
   int solve(double* X, int msize, const std::function<void(const double*, double*)>& residual,
@@ -45,6 +69,7 @@ class Ngmres {
   hipStream_t st;
   double *Fv = nullptr, *XM = nullptr, *FM = nullptr, *XA = nullptr, *FA = nullptr;
   double *partial = nullptr, *small = nullptr;
+  KTimer* kt = nullptr;
 
  private:
   std::vector<double*> bufs, hist;
@@ -96,6 +121,7 @@ class Rve {
   long long* d_minmax_i = nullptr;
   double *d_u = nullptr, *d_xp = nullptr, *d_xn = nullptr, *d_fsum = nullptr;  // coordinates
   double* d_n2 = nullptr;  // per-cell |dPdF|^2
+  double* d_scr_partial = nullptr;  // partial sums of the iteration counters
 
   // MechBase state (mech_base.h:75-107), column-major
   double C_ref[81], C_volAvg[81], C_minMaxAvg[81], C_volAvgLastInc[81], C_minMaxAvgLastInc[81], S[81];
@@ -108,6 +134,7 @@ class Rve {
   bool terminally_ill = false;
   int snes_iter = 0, nfuncs = 0, solve_evals = 0;
   sc_stats stats;
+  KTimer kt;
 
  private:
   std::vector<double*> allocs;

@@ -207,6 +207,52 @@ void sym_lstsq(int l, std::vector<double> H, std::vector<double>& b) {
 }
 
 // ---------------------------------------------------------------------------
+hipEvent_t KTimer::get() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  SCA_HIP(hipEventCreate(&e));
+  return e;
+}
+void KTimer::begin(int, hipStream_t st) {
+  if (!on) return;
+  open_a = get();
+  SCA_HIP(hipEventRecord(open_a, st));
+}
+void KTimer::end(int cls, int64_t points, hipStream_t st) {
+  if (!on || !open_a) return;
+  hipEvent_t b = get();
+  SCA_HIP(hipEventRecord(b, st));
+  pending.push_back({cls, points, open_a, b});
+  open_a = nullptr;
+}
+void KTimer::flush() {
+  if (pending.empty()) return;
+  SCA_HIP(hipEventSynchronize(pending.back().b));
+  for (const Rec& r : pending) {
+    float ms = 0.0f;
+    SCA_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    acc.ms[r.cls] += ms;
+    acc.launches[r.cls] += 1;
+    acc.points[r.cls] += r.pts;
+    pool.push_back(r.a);
+    pool.push_back(r.b);
+  }
+  pending.clear();
+}
+KTimer::~KTimer() {
+  for (const Rec& r : pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  if (open_a) (void)hipEventDestroy(open_a);
+}
+
+// ---------------------------------------------------------------------------
 double* Rve::dalloc(size_t n) {
   double* p = nullptr;
   SCA_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)));
@@ -247,7 +293,7 @@ Rve::Rve(const sc_rve_desc& d) {
   build_matconst(d.material, d.phase_num, mc);
   SCA_HIP(hipStreamCreate(&st));
   fft.init(cells);
-  ngm = new Ngmres(9 * N, st);
+  ngm = new Ngmres(9 * N, st, &kt);
   // material point planes
   mp.N = N;
   mp.F = dalloc(9 * N);
@@ -279,6 +325,7 @@ Rve::Rve(const sc_rve_desc& d) {
   d_xn = dalloc((size_t)3 * (cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1));
   d_fsum = dalloc(9);
   d_n2 = dalloc(N);
+  d_scr_partial = dalloc((size_t)2 * red_blocks());
   SCA_HIP(hipMalloc(&d_ill, sizeof(int)));
   SCA_HIP(hipMalloc(&d_minmax_i, 2 * red_blocks() * sizeof(long long)));
   if (d.grid_num.derivative != 0) {
@@ -379,9 +426,19 @@ std::vector<double> Rve::reduce(const std::vector<int>& ops, const std::vector<c
 void Rve::constitutive_response(const double* Fdev, double dt) {
   SCA_HIP(hipMemcpyAsync(mp.F, Fdev, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
   SCA_HIP(hipMemsetAsync(d_ill, 0, sizeof(int), st));
+  kt.begin(SC_KT_RESPONSE, st);
   launch_mp_response(mp, mc, dt, d_ill, sub, st);
+  kt.end(SC_KT_RESPONSE, N, st);
+  kt.begin(SC_KT_REDUCE, st);
   launch_pav_partial(mp.P, N, partial, st);
   launch_final_reduce(partial, 9, mp_partial_blocks(N), small + SM_SUMS, st);
+  kt.end(SC_KT_REDUCE, N, st);
+  if (kt.on) {  // iteration counts of this evaluation
+    const int ops[2] = {2, 2};
+    const int* a[2] = {mp.iters, mp.iters + N};
+    launch_partial_reduce<int>(ops, N, 2, a, nullptr, d_scr_partial, st);
+    launch_final_reduce(d_scr_partial, 2, red_blocks(), small + SM_ITERS, st);
+  }
   last_dt = dt;
 }
 
@@ -396,12 +453,21 @@ void Rve::fetch_response_scalars(bool rotate) {
   if (rotate) h::rot2(rot_q, P_av, P_av);
   last_div_sum = h_s[9];
   if (ill) terminally_ill = true;
+  if (kt.on) {
+    double it[2];
+    SCA_HIP(hipMemcpy(it, small + SM_ITERS, sizeof it, hipMemcpyDeviceToHost));
+    stats.newton_iterations += (int64_t)it[0];
+    stats.state_iterations += (int64_t)it[1];
+  }
+  kt.flush();
 }
 
 // consistent tangent of the last evaluation: C_volAvg, C_minMaxAvg
 // (spectral.cpp:189-217)
 void Rve::tangent() {
+  kt.begin(SC_KT_TANGENT, st);
   launch_mp_tangent(mp, mc, last_dt, dPdF, st);
+  kt.end(SC_KT_TANGENT, N, st);
   std::vector<int> ops(81, 2);
   std::vector<const double*> a(81);
   for (int q = 0; q < 81; ++q) a[q] = dPdF + q * N;
@@ -424,6 +490,7 @@ void Rve::tangent() {
   }
   SCA_HIP(hipStreamSynchronize(st));
   for (int q = 0; q < 81; ++q) C_minMaxAvg[q] = 0.5 * (dmax[q] + dmin[q]);
+  kt.flush();
 }
 
 GammaHost Rve::gamma_host(bool do_gamma) const {
@@ -454,10 +521,16 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Rrm[3 * i + j] = R[i + 3 * j];
   launch_bc_aim(small + SM_SUMS, S, Rrm, wgt, 1, small + SM_AIM, st);
+  kt.begin(SC_KT_FFT_FWD, st);
   fft.forward(mp.P, spec, 9, st);
+  kt.end(SC_KT_FFT_FWD, N, st);
+  kt.begin(SC_KT_GAMMA, st);
   launch_gamma_div<double>(spec, K, gamma_host(true), small + SM_AIM, d_xi_tab, partial, st);
   launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
+  kt.end(SC_KT_GAMMA, K, st);
+  kt.begin(SC_KT_FFT_BWD, st);
   fft.backward(spec, rdev, 9, 1.0, st);
+  kt.end(SC_KT_FFT_BWD, N, st);
   fetch_response_scalars(true);
   err_div = std::sqrt(last_div_sum) * wgt;
   if (cells[0] == 1) err_div = err_div / 2;
@@ -489,7 +562,7 @@ int Rve::converged() const {
 // ---------------------------------------------------------------------------
 // SNESNGMRES on device vectors of length n
 // ---------------------------------------------------------------------------
-Ngmres::Ngmres(int64_t n_, hipStream_t st_) : n(n_), st(st_) {
+Ngmres::Ngmres(int64_t n_, hipStream_t st_, KTimer* kt_) : n(n_), st(st_), kt(kt_) {
   auto a = [&](double*& p, size_t m) {
     SCA_HIP(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(double)));
     bufs.push_back(p);
@@ -520,12 +593,15 @@ std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vecto
   std::vector<double> out(nvec);
   for (int v0 = 0; v0 < nvec; v0 += MAX_RED_VEC) {
     const int nv = std::min(MAX_RED_VEC, nvec - v0);
+    if (kt) kt->begin(SC_KT_REDUCE, st);
     launch_partial_reduce<double>(ops.data() + v0, n, nv, a.data() + v0, b.empty() ? nullptr : b.data() + v0,
                                   partial, st);
     launch_final_reduce(partial, nv, red_blocks(), small, st);
+    if (kt) kt->end(SC_KT_REDUCE, n * nv, st);
     SCA_HIP(hipMemcpyAsync(out.data() + v0, small, nv * sizeof(double), hipMemcpyDeviceToHost, st));
     SCA_HIP(hipStreamSynchronize(st));
   }
+  if (kt) kt->flush();
   return out;
 }
 
@@ -553,7 +629,9 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
   update_subspace(0, Fv, fnorm, X);
   int k_restart = 1, l = 1, ivec = 0, restart_count = 0;
   for (int k = 1; k < max_its + 1; ++k) {
+    if (kt) kt->begin(SC_KT_REDUCE, st);
     launch_axpby(1.0, X, -1.0, Fv, XM, n, st);  // basic line search: XM = X - F
+    if (kt) kt->end(SC_KT_REDUCE, n, st);
     residual(XM, FM);
     // ||FM||^2, <FM, Fdot_i>, <Fdot_ivec, Fdot_i> in one pass
     std::vector<int> ops(1 + 2 * l, 0);
@@ -590,7 +668,9 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       for (int i = 0; i < l; ++i) alph_total += beta[i];
       std::vector<const double*> xs(l);
       for (int i = 0; i < l; ++i) xs[i] = Xdot(i);
+      if (kt) kt->begin(SC_KT_REDUCE, st);
       launch_lincomb(XM, 1.0 - alph_total, l, beta.data(), xs.data(), XA, n, st);
+      if (kt) kt->end(SC_KT_REDUCE, n * (l + 1), st);
       residual(XA, FA);
     }
     if (fminnorm > fMnorm) fminnorm = fMnorm;

@@ -805,6 +805,8 @@ template void launch_partial_reduce<double>(const int*, int64_t, int, const doub
                                             double*, hipStream_t);
 template void launch_partial_reduce<float>(const int*, int64_t, int, const float* const*, const float* const*,
                                            double*, hipStream_t);
+template void launch_partial_reduce<int>(const int*, int64_t, int, const int* const*, const int* const*, double*,
+                                         hipStream_t);
 
 // Cells whose tangents form C_minMaxAvg (spectral.cpp:189-207): the first
 // cell of largest and of smallest |dPdF|^2, where norms within a relative

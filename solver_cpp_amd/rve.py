@@ -13,8 +13,8 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from .capi import (SC_DEF, SC_FIELD, SC_STRESS, GridNumerics, Library, LoadStep, Material, PhaseNumerics,
-                   RveDesc, Solution, SolverScalars, Stats, ptr)
+from .capi import (KT_CLASSES, SC_DEF, SC_FIELD, SC_STRESS, GridNumerics, KernelTimes, Library, LoadStep, Material,
+                   PhaseNumerics, RveDesc, Solution, SolverScalars, Stats, ptr)
 
 
 @dataclass
@@ -167,3 +167,63 @@ class Rve:
 
     def reset_stats(self):
         self.lib.check(self.lib.sc_rve_reset_stats(self.h), "reset_stats")
+
+    # measurement (HIP library) -------------------------------------------------
+    def profile(self, enable: bool = True):
+        self.lib.check(self.lib.sc_rve_profile(self.h, int(enable)), "profile")
+
+    def kernel_times(self) -> dict:
+        k = KernelTimes()
+        self.lib.check(self.lib.sc_rve_get_kernel_times(self.h, C.byref(k)), "kernel_times")
+        return {name: dict(launches=int(k.launches[i]), ms=float(k.ms[i]), points=int(k.points[i]))
+                for i, name in enumerate(KT_CLASSES)}
+
+    # increment loop -------------------------------------------------------------
+    def increments(self, loads: Sequence[Load], max_cut_back: Optional[int] = None):
+        """DamaskGrid::loop_over_loadcase (damask_grid.cpp:82-196) driven from
+        the host one increment at a time: yields one dict per converged
+        increment (index, time, iterations, evaluations, cut-backs, wall
+        seconds).  Same control flow as sc_rve_run_loadcase."""
+        import time as _time
+        if max_cut_back is None:
+            max_cut_back = int(self.desc.grid_num.max_cut_back)
+        t, dt, dt_prev = 0.0, 1.0, 0.0
+        cut_back, cut_back_level, total_inc = False, 0, 0
+        for l, load in enumerate(loads):
+            t_0 = t
+            guess = bool(load.estimate_rate) and l > 0
+            for inc in range(int(load.N)):
+                total_inc += 1
+                w0 = _time.perf_counter()
+                dt_prev = dt
+                if abs(load.r - 1.0) < 1e-9:
+                    dt = load.t / load.N
+                else:
+                    dt = load.t * (load.r ** inc - load.r ** (inc + 1)) / (1.0 - load.r ** load.N)
+                dt *= 2.0 ** (-cut_back_level)
+                step_fraction, its, evs, cuts = 0, 0, 0, 0
+                while step_fraction < 2 ** cut_back_level:
+                    t_remaining = load.t + t_0 - t
+                    t += dt
+                    step_fraction += 1
+                    self.forward(cut_back, guess, dt, dt_prev, t_remaining, load)
+                    sol = self.calculate_solution()
+                    its += sol.required_iterations
+                    evs += sol.residual_evaluations
+                    if sol.converged and sol.stag_converged and not sol.terminally_ill:
+                        self.update_coords()
+                        dt_prev = dt
+                        cut_back = False
+                        guess = True
+                    elif cut_back_level < max_cut_back:
+                        cut_back = True
+                        step_fraction = (step_fraction - 1) * 2
+                        cut_back_level += 1
+                        t -= dt
+                        dt /= 2.0
+                        cuts += 1
+                    else:
+                        raise RuntimeError(f"no solution found for increment {total_inc}")
+                cut_back_level = max(0, int(cut_back) - 1)
+                yield dict(increment=total_inc, t=t, iterations=its, evaluations=evs, cutbacks=cuts,
+                           seconds=_time.perf_counter() - w0)
