@@ -27,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from solver_cpp_amd.capi import Library  # noqa: E402
-from solver_cpp_amd.geometry import polycrystal  # noqa: E402
+from solver_cpp_amd.geometry import CONFIGS, EXAMPLE_NUMERICS, polycrystal  # noqa: E402
 from solver_cpp_amd.rve import Rve, tension_x  # noqa: E402
 
 PRODUCT_SO = os.path.join(ROOT, "solver_cpp_amd", "_build", "libsolver_cpp_amd.so")
@@ -37,7 +37,7 @@ METRIC = "mat-point updates/s @128^3 vs plain FP64 GPU; p50/p99 increment latenc
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic work per unit, per kernel class ("Roofline"):
+# Algorithmic work per unit, per kernel class:
 #   response: bytes of the state read and written per cell and evaluation
 #             (F F0 Fp0 Lp S0 xi0 gamma0 in, Lp S P Fp Fe xi gamma out, 2 counters)
 #             and FP64 flops per Newton iteration / state iteration / cell
@@ -60,7 +60,7 @@ def run_gpu(args, loads, mat, ori, size):
     if lib.backend != "hip-gfx950":
         raise RuntimeError(f"unexpected backend {lib.backend}")
     cells = (args.cells,) * 3
-    rve = Rve(lib, cells, size, mat, ori, precision=1 if args.precision == "mixed" else 0)
+    rve = Rve(lib, cells, size, mat, ori, precision=1 if args.precision == "mixed" else 0, **EXAMPLE_NUMERICS)
     t0 = time.perf_counter()
     rve.init()
     log(f"[bench] init {cells} with {args.grains} grains: {time.perf_counter() - t0:.2f} s")
@@ -117,8 +117,8 @@ def cpu_baseline(args, loads):
     lib = Library(ORACLE_SO)
     c = args.cpu_sample_cells
     grains = max(2, round(args.grains * (c / args.cells) ** 3))
-    mat, ori, size = polycrystal((c, c, c), grains, seed=args.seed)
-    rve = Rve(lib, (c, c, c), size, mat, ori)
+    mat, ori, size = polycrystal((c, c, c), grains, seed=args.seed, ori_seed=args.ori_seed)
+    rve = Rve(lib, (c, c, c), size, mat, ori, **EXAMPLE_NUMERICS)
     rve.init()
     it = rve.increments(loads)
     t0 = time.perf_counter()
@@ -140,19 +140,21 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5, help="timed increments")
     ap.add_argument("--warmup", type=int, default=2, help="untimed increments before the timed ones")
-    ap.add_argument("--cells", type=int, default=128)
-    ap.add_argument("--grains", type=int, default=1000)
-    ap.add_argument("--increments", type=int, default=60, help="increments of the load case (3 %% strain)")
+    c3 = CONFIGS["C3"]
+    ap.add_argument("--cells", type=int, default=c3["edge"])
+    ap.add_argument("--grains", type=int, default=c3["grains"])
+    ap.add_argument("--increments", type=int, default=c3["N"], help="increments of the load case (3 %% strain)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=c3["seed"], help="Voronoi seed")
+    ap.add_argument("--ori-seed", type=int, default=c3["ori_seed"], help="orientation seed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cells", type=int, default=32)
     ap.add_argument("--cpu-sample-increments", type=int, default=2)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU: the solver path is built and measured on a single MI355X "
-              "(sharding across GPUs is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py runs on one GPU: the solver path is built and measured on a single MI355X; "
+              "several GPUs are not supported by this benchmark.", file=sys.stderr)
         sys.exit(2)
     if args.warmup + args.steps > args.increments:
         sys.exit("warmup + steps exceeds the increments of the load case")
@@ -161,7 +163,7 @@ def main():
     t_total = 0.03 / rate  # 3 % strain
     loads = [tension_x(t=t_total, N=args.increments, rate=rate)]
     cells = (args.cells,) * 3
-    mat, ori, size = polycrystal(cells, args.grains, seed=args.seed)
+    mat, ori, size = polycrystal(cells, args.grains, seed=args.seed, ori_seed=args.ori_seed)
     res = run_gpu(args, loads, mat, ori, size)
     value = res["N"] * res["evals"] / res["wall"]
     lat_ms = np.array(res["lat"]) * 1e3
@@ -179,9 +181,9 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"spectral basic grid solver, {args.cells}^3 periodic grid, {args.grains} Voronoi grains "
-                        f"(seed {args.seed}), phenopowerlaw FCC, uniaxial tension dot_F_xx = 1e-3/s, "
-                        f"{args.increments} increments to 3 % strain",
+            "workload": f"spectral basic grid solver, {args.cells}^3 periodic grid (1e-5 m cube), {args.grains} "
+                        f"Voronoi grains (seeds {args.seed}/{args.ori_seed}), phenopowerlaw FCC, uniaxial tension "
+                        f"dot_F_xx = 1e-3/s, {args.increments} increments to 3 % strain, N_iter_min/max = 4/40",
             "cells": list(cells),
             "grains": args.grains,
             "precision": args.precision,

@@ -349,6 +349,42 @@ void f_discretization_init(int* material_at, int* n_cells, double* ip0, int* n_i
                            double* node0, int* n_node, int* shared_nodes_begin);
 void f_discretization_fetch_ip_node_coord_pointers(double** ip, double** node);
 
+/* VTI.f90:146-202 and :473-497 (bind(C) names of the reference's VTI
+ * reader, called by discretization_grid.h:64-76): cells, size = spacing *
+ * cells and origin of a VTK image-data file held in memory, and an integer
+ * cell dataset returned +1 (1-based material entries).  Errors leave
+ * cells = -1 / dataset_size = -1 and set sc_last_error(). */
+void f_VTI_readCellsSizeOrigin(const char* content, int length, int* cells, double* size, double* origin);
+void f_VTI_readDataset_int(const char* content, int length, const char* label, int label_length,
+                           int* dataset, int* dataset_size);
+
+/* Boundary state for tests: resident constitutive fields of the cells
+ * (SC_FIELD_FP, _LP, _S, _FE, _P, _XI, _GAMMA; (3,3,N) or (12,N)), the number
+ * of f_materialpoint_result calls, and a reset of the process-global state.
+ * No counterpart in the reference. */
+int  sc_boundary_get_state(int field, double* out, int64_t n_values);
+int  sc_boundary_result_count(void);
+void sc_boundary_reset(void);
+
+/* ------------------------------------------------------------------------- */
+/* Job inputs (host)                                                          */
+/* ------------------------------------------------------------------------- */
+
+/* Geometry file: cells, size, origin and the 'material' cell dataset as
+ * 0-based material indices (material_at may be NULL to query the cells). */
+int sc_read_vti(const char* path, int cells[3], double size[3], double origin[3], int* material_at,
+                int64_t capacity);
+/* material.yaml with one cF phenopowerlaw phase and one constituent per
+ * material entry (material.f90:84-205): constants and the (4, n) entry
+ * orientations (orientations may be NULL to query n). */
+int sc_read_material(const char* path, sc_material* m, double* orientations, int capacity, int* n_materials);
+/* numerics.yaml (config.cpp:362-460, phase_mechanical.f90:280-294): updates
+ * the given structs (initialise them with the defaults first). */
+int sc_read_numerics(const char* path, sc_grid_numerics* g, sc_phase_numerics* p);
+/* Load-case YAML (config.cpp:240-360): loadsteps in file order (steps may be
+ * NULL to query the count).  t, N and r are read as integers, as there. */
+int sc_read_load(const char* path, sc_loadstep* steps, int capacity, int* n_steps);
+
 /* math.f90:599 (n x n inverse, err = 1 on failure), :819, :849, :428 */
 void f_math_invert(double* invA, int* err, double* A, int* n);
 void f_math_3333to99(double* m3333, double* m99);

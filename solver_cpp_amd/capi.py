@@ -151,7 +151,8 @@ PROFILE_SIGNATURES = {
     "sc_rve_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
 }
 
-# Symbols only the product exports (drop-in boundary of the Fortran core).
+# Symbols only the product exports: the drop-in boundary of the Fortran core
+# (f_*) and the job-input readers.
 BOUNDARY_SIGNATURES = {
     "f_materialpoint_initBase": (None, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                         C.c_char_p, C.c_int]),
@@ -164,6 +165,15 @@ BOUNDARY_SIGNATURES = {
     "f_homogenization_mechanical_response2": (None, [_dp, _ip, _ip]),
     "f_discretization_init": (None, [_ip, _ip, _dp, _ip, _dp, _ip, _ip]),
     "f_discretization_fetch_ip_node_coord_pointers": (None, [C.POINTER(_dp), C.POINTER(_dp)]),
+    "f_VTI_readCellsSizeOrigin": (None, [C.c_char_p, C.c_int, _ip, _dp, _dp]),
+    "f_VTI_readDataset_int": (None, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, _ip, _ip]),
+    "sc_boundary_get_state": (C.c_int, [C.c_int, _dp, C.c_int64]),
+    "sc_boundary_result_count": (C.c_int, []),
+    "sc_boundary_reset": (None, []),
+    "sc_read_vti": (C.c_int, [C.c_char_p, _ip, _dp, _dp, _ip, C.c_int64]),
+    "sc_read_material": (C.c_int, [C.c_char_p, C.POINTER(Material), _dp, C.c_int, _ip]),
+    "sc_read_numerics": (C.c_int, [C.c_char_p, C.POINTER(GridNumerics), C.POINTER(PhaseNumerics)]),
+    "sc_read_load": (C.c_int, [C.c_char_p, C.POINTER(LoadStep), C.c_int, _ip]),
 }
 
 

@@ -69,13 +69,14 @@ struct MpState {
 };
 
 // Kernel launchers (constitutive.hip)
+// cells [c_begin, c_end) of the planes (c_end < 0: all N)
 void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_ill, double* d_sub, hipStream_t st,
-                        int* d_broken = nullptr);
+                        int* d_broken = nullptr, int64_t c_begin = 0, int64_t c_end = -1);
 int mp_partial_blocks(int64_t N);
 int mp_sub_planes();
 void launch_pav_partial(const double* P, int64_t N, double* partial, hipStream_t st);
 void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF /*(81,N) SoA*/,
-                       hipStream_t st);
+                       hipStream_t st, int64_t c_begin = 0, int64_t c_end = -1);
 void launch_mp_forward(const MpState& s, hipStream_t st);
 void launch_mp_init(const MpState& s, const double* d_Fp_init /*(9,N)*/, double xi0, hipStream_t st);
 

@@ -51,6 +51,7 @@ struct KernelArgs {
   int* ill;
   int* broken;  // per point, may be null
   double* sub;  // substep scratch, SUB_PLANES planes of N doubles
+  int64_t c_begin, c_end;  // cells [c_begin, c_end) of the N-cell planes
 };
 
 __constant__ MatConst g_mc;
@@ -473,7 +474,7 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
   const MatConst& mc = g_mc;
   const MpState& s = a.s;
   const int64_t N = s.N;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   // FPI history (dotState_last(:,1), its difference to dotState_last(:,2))
   // and the accumulated-shear half of the state stay in LDS while the Lp
   // Newton iteration needs the registers
@@ -481,7 +482,7 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
   __shared__ double h_d12[NSTATE][64];
   __shared__ double h_gam[NSLIP][64];
   const int t = threadIdx.x;
-  if (c >= N) return;
+  if (c >= a.c_end) return;
   const PointIO io{&s, N, c};
   double* sub = a.sub;
   double Lp[9], S[9];
@@ -677,6 +678,7 @@ struct TangentArgs {
   double dt;
   double* dPdF;
   double C[81];  // C(i,j,k,l) at 27i+9j+3k+l (row-major)
+  int64_t c_begin, c_end;
 };
 
 __device__ __forceinline__ double Cijkl(const TangentArgs& a, int i, int j, int k, int l) {
@@ -688,8 +690,8 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
   const MatConst& mc = g_mc;
   const MpState& s = a.s;
   const int64_t N = s.N;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
+  const int64_t c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.c_end) return;
   const double dt = a.dt;
   double F[9], Fp[9], Fp0[9], S[9], Fe[9], xi[NSLIP];
   load33(s.F, N, c, F);
@@ -962,10 +964,13 @@ int mp_partial_blocks(int64_t N) { return (int)((N + 255) / 256); }
 int mp_sub_planes() { return SUB_PLANES; }
 
 void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_ill, double* d_sub, hipStream_t st,
-                        int* d_broken) {
+                        int* d_broken, int64_t c_begin, int64_t c_end) {
+  if (c_end < 0) c_end = s.N;
+  if (c_begin < 0 || c_end > s.N || c_begin > c_end) throw std::runtime_error("material-point range out of bounds");
+  if (c_begin == c_end) return;
   SCA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mc), &mc, sizeof(MatConst), 0, hipMemcpyHostToDevice, st));
-  KernelArgs a{s, dt, d_ill, d_broken, d_sub};
-  const int blocks = (int)((s.N + 63) / 64);
+  KernelArgs a{s, dt, d_ill, d_broken, d_sub, c_begin, c_end};
+  const int blocks = (int)((c_end - c_begin + 63) / 64);
   // exponents of the example material (n_sl = 20, a_sl = 2.25) are compiled
   // in; any other material runs the runtime-exponent instantiation
   if (mc.n_e4 == 80 && mc.a_e4 == 9)
@@ -980,12 +985,18 @@ void launch_pav_partial(const double* P, int64_t N, double* partial, hipStream_t
   SCA_HIP(hipGetLastError());
 }
 
-void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF, hipStream_t st) {
+void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF, hipStream_t st,
+                       int64_t c_begin, int64_t c_end) {
+  if (c_end < 0) c_end = s.N;
+  if (c_begin < 0 || c_end > s.N || c_begin > c_end) throw std::runtime_error("material-point range out of bounds");
+  if (c_begin == c_end) return;
   SCA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mc), &mc, sizeof(MatConst), 0, hipMemcpyHostToDevice, st));
   TangentArgs a;
   a.s = s;
   a.dt = dt;
   a.dPdF = dPdF;
+  a.c_begin = c_begin;
+  a.c_end = c_end;
   // cubic stiffness tensor from C11, C12, C44 (math_Voigt66to3333_stiffness)
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
@@ -996,7 +1007,7 @@ void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* 
           else if ((i == k && j == l) || (i == l && j == k)) v = mc.C44;
           a.C[27 * i + 9 * j + 3 * k + l] = v;
         }
-  const int blocks = (int)((s.N + 63) / 64);
+  const int blocks = (int)((c_end - c_begin + 63) / 64);
   hipLaunchKernelGGL(mp_tangent_kernel, dim3(blocks), dim3(64), 0, st, a);
   SCA_HIP(hipGetLastError());
 }

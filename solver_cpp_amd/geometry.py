@@ -49,10 +49,25 @@ def uniform_orientations(n: int, seed: int = 0) -> np.ndarray:
     return np.ascontiguousarray(q / np.linalg.norm(q, axis=1, keepdims=True))
 
 
-def polycrystal(cells, n_grains: int, seed: int = 0, size=None):
-    """(material_at, orientations, size) of a seeded periodic polycrystal; the
-    default size is one unit length per cell edge scaled to 1e-5 m (the grid
-    examples' 16 cells over 1e-5 m per edge)."""
+def polycrystal(cells, n_grains: int, seed: int = 0, size=None, ori_seed=None):
+    """(material_at, orientations, size) of a seeded periodic polycrystal in a
+    1e-5 m cube (the grid examples' box) unless ``size`` is given.  Seeds:
+    ``seed`` for the Voronoi points, ``ori_seed`` (default seed + 1) for the
+    orientations."""
     if size is None:
-        size = [1.0e-5 * c / 16.0 for c in cells]
-    return voronoi(cells, size, n_grains, seed), uniform_orientations(n_grains, seed + 7919), list(size)
+        size = [1.0e-5, 1.0e-5, 1.0e-5]
+    if ori_seed is None:
+        ori_seed = seed + 1
+    return voronoi(cells, size, n_grains, seed), uniform_orientations(n_grains, ori_seed), list(size)
+
+
+# The measured configurations: grid edge, grains, Voronoi seed, orientation seed,
+# load-case duration (s) and increments (tension at dot_F_xx = 1e-3/s, dt = 0.5 s)
+CONFIGS = {
+    "C1": dict(edge=16, grains=20, seed=1, ori_seed=2, t=10.0, N=20),
+    "C2": dict(edge=64, grains=200, seed=3, ori_seed=4, t=20.0, N=40),
+    "C3": dict(edge=128, grains=1000, seed=5, ori_seed=6, t=30.0, N=60),
+}
+
+# examples/grid/numerics.yaml: N_iter_min = 4, N_iter_max = 40
+EXAMPLE_NUMERICS = dict(itmin=4, itmax=40)

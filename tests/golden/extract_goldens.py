@@ -215,6 +215,40 @@ def main():
                         increment=np.array(inc_of_eval, dtype=np.int32),
                         converged_increments=np.array(conv_incs, dtype=np.int32))
     print(f"wrote {len(out)} golden groups, {len(evals_F)} trace evaluations")
+    write_job_inputs()
+
+
+def write_job_inputs():
+    """Input data of the reference's grid example and geometry fixture:
+    - grid16_20grains.vti: the 16^3 VTK image data held as a string by the
+      reference's VTISetup fixture (test_discretization_grid.cpp);
+    - material.yaml / numerics.yaml / tensionX.yaml: the values of
+      examples/grid/{material,numerics,tensionX}.yaml, re-emitted from the
+      parsed data (not the original text)."""
+    import yaml
+
+    src = read("test_discretization_grid.cpp")
+    a = src.index('R""""(') + len('R""""(')
+    b = src.index(')""""', a)
+    vti = src[a:b].strip() + "\n"
+    with open(os.path.join(HERE, "grid16_20grains.vti"), "w") as f:
+        f.write(vti)
+    ex = os.path.join(REF, "examples", "grid")
+
+    def load(name):
+        with open(os.path.join(ex, name)) as f:
+            return yaml.safe_load(f)
+
+    def emit(name, data, header):
+        with open(os.path.join(HERE, name), "w") as f:
+            f.write(header)
+            yaml.safe_dump(data, f, default_flow_style=None, sort_keys=False, width=100)
+
+    emit("material.yaml", load("material.yaml"),
+         "# values of the grid example's material.yaml: one cF phenopowerlaw phase, 20 entries\n")
+    emit("numerics.yaml", load("numerics.yaml"), "# values of the grid example's numerics.yaml\n")
+    emit("tensionX.yaml", load("tensionX.yaml"), "# values of the grid example's tensionX.yaml\n")
+    print("wrote grid16_20grains.vti, material.yaml, numerics.yaml, tensionX.yaml")
 
 
 if __name__ == "__main__":

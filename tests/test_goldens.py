@@ -179,7 +179,7 @@ def test_ngmres_replays_fortran_trace(lib, trace):
         rc = lib.sc_ngmres_replay(243, 30, ptr(x0), len(idx), ptr(rr), ptr(xe))
         assert rc == 0
         np.testing.assert_allclose(xe, F[idx], atol=1e-14, rtol=0)
-    # iterations per increment (SURVEY counts total_iter values 0..k)
+    # iterations per increment: total_iter runs 0..k over 2k+1 evaluations
     counts = [(np.sum(inc == i) - 1) // 2 + 1 for i in range(1, 11)]
     assert counts == [4, 5, 8, 7, 7, 7, 6, 6, 5, 4]
 

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from solver_cpp_amd.capi import ptr
-from solver_cpp_amd.geometry import polycrystal, uniform_orientations
+from solver_cpp_amd.geometry import CONFIGS, EXAMPLE_NUMERICS, polycrystal, uniform_orientations
 from solver_cpp_amd.rve import Rve, tension_x
 
 pytestmark = pytest.mark.gpu
@@ -203,8 +203,8 @@ def test_ngmres_replays_fortran_trace(product, trace):
 
 
 # ---- full solver ------------------------------------------------------------------------
-def _pair(product, oracle, cells, grains, seed=1, **kw):
-    mat, ori, size = polycrystal(cells, grains, seed=seed)
+def _pair(product, oracle, cells, grains, seed=1, ori_seed=None, **kw):
+    mat, ori, size = polycrystal(cells, grains, seed=seed, ori_seed=ori_seed)
     return Rve(product, cells, size, mat, ori, **kw), Rve(oracle, cells, size, mat, ori, **kw)
 
 
@@ -233,12 +233,15 @@ def test_init_and_single_residual(product, oracle):
 
 def test_loadcase_16cubed_20grains(product, oracle):
     """Plain FP64 on the CPU-runnable configuration: 16^3 grid, 20 Voronoi
-    grains, uniaxial tension dot_F_xx = 1e-3/s to 1 % strain in 20 increments.
-    Per-point P and F within 1e-10, identical iteration counts per increment."""
-    a, b = _pair(product, oracle, (16, 16, 16), 20)
+    grains, uniaxial tension dot_F_xx = 1e-3/s to 1 % strain in 20 increments,
+    N_iter_min/max = 4/40.  Per-point P and F within 1e-10, identical
+    iteration and evaluation counts per increment."""
+    cfg = CONFIGS["C1"]
+    e = cfg["edge"]
+    a, b = _pair(product, oracle, (e, e, e), cfg["grains"], cfg["seed"], cfg["ori_seed"], **EXAMPLE_NUMERICS)
     a.init()
     b.init()
-    load = [tension_x(t=10.0, N=20)]
+    load = [tension_x(t=cfg["t"], N=cfg["N"])]
     ia, ea = a.loop_over_loadcase(load)
     ib, eb = b.loop_over_loadcase(load)
     np.testing.assert_array_equal(ia, ib)
