@@ -55,15 +55,15 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(args, loads, mat, ori, size):
+def run_gpu(args, loads, mat, ori, size, precision):
     lib = Library(PRODUCT_SO)
     if lib.backend != "hip-gfx950":
         raise RuntimeError(f"unexpected backend {lib.backend}")
     cells = (args.cells,) * 3
-    rve = Rve(lib, cells, size, mat, ori, precision=1 if args.precision == "mixed" else 0, **EXAMPLE_NUMERICS)
+    rve = Rve(lib, cells, size, mat, ori, precision=1 if precision == "mixed" else 0, **EXAMPLE_NUMERICS)
     t0 = time.perf_counter()
     rve.init()
-    log(f"[bench] init {cells} with {args.grains} grains: {time.perf_counter() - t0:.2f} s")
+    log(f"[bench] {precision}: init {cells} with {args.grains} grains: {time.perf_counter() - t0:.2f} s")
     it = rve.increments(loads)
     for _ in range(args.warmup):
         r = next(it)
@@ -84,7 +84,10 @@ def run_gpu(args, loads, mat, ori, size):
     st = rve.stats()
     rve.profile(False)
     N = int(np.prod(cells))
-    return dict(N=N, wall=wall, lat=lat, evals=evals, kt=kt, stats=st, P_av=list(rve.scalars().P_av))
+    out = dict(N=N, wall=wall, lat=lat, evals=evals, kt=kt, stats=st, P_av=np.array(rve.scalars().P_av),
+               P=rve.field("P"))
+    rve.close()
+    return out
 
 
 def roofline(res):
@@ -144,7 +147,8 @@ def main():
     ap.add_argument("--cells", type=int, default=c3["edge"])
     ap.add_argument("--grains", type=int, default=c3["grains"])
     ap.add_argument("--increments", type=int, default=c3["N"], help="increments of the load case (3 %% strain)")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                    help="measured form; the plain FP64 form runs the same increments for the comparison")
     ap.add_argument("--seed", type=int, default=c3["seed"], help="Voronoi seed")
     ap.add_argument("--ori-seed", type=int, default=c3["ori_seed"], help="orientation seed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -164,9 +168,12 @@ def main():
     loads = [tension_x(t=t_total, N=args.increments, rate=rate)]
     cells = (args.cells,) * 3
     mat, ori, size = polycrystal(cells, args.grains, seed=args.seed, ori_seed=args.ori_seed)
-    res = run_gpu(args, loads, mat, ori, size)
+    plain = run_gpu(args, loads, mat, ori, size, "fp64")
+    res = plain if args.precision == "fp64" else run_gpu(args, loads, mat, ori, size, args.precision)
     value = res["N"] * res["evals"] / res["wall"]
+    plain_value = plain["N"] * plain["evals"] / plain["wall"]
     lat_ms = np.array(res["lat"]) * 1e3
+    plain_ms = np.array(plain["lat"]) * 1e3
     line = {
         "metric": METRIC,
         "value": value,
@@ -191,6 +198,16 @@ def main():
         },
         "p50_increment_ms": float(np.percentile(lat_ms, 50)),
         "p99_increment_ms": float(np.percentile(lat_ms, 99)),
+        "plain_fp64": {"value": plain_value, "ms_per_step": float(plain_ms.mean()),
+                       "p50_increment_ms": float(np.percentile(plain_ms, 50)),
+                       "p99_increment_ms": float(np.percentile(plain_ms, 99)),
+                       "residual_evaluations": plain["evals"]},
+        "speedup_vs_plain_fp64": value / plain_value,
+        "increment_time_ratio_plain_over_this": float(plain_ms.mean() / lat_ms.mean()),
+        "error_vs_plain_fp64": {
+            "P_av_rel": float(np.abs(res["P_av"] - plain["P_av"]).max() / np.abs(plain["P_av"]).max()),
+            "P_max_rel": float(np.abs(res["P"] - plain["P"]).max() / np.abs(plain["P"]).max())},
+        "fp32_evaluations": int(res["stats"].fp32_evaluations),
         "residual_evaluations": res["evals"],
         "newton_iterations": int(res["stats"].newton_iterations),
         "state_iterations": int(res["stats"].state_iterations),

@@ -126,6 +126,7 @@ typedef struct sc_stats {
   int64_t newton_iterations;      /* sum over points and evaluations (if counted) */
   int64_t state_iterations;       /* sum over points and evaluations (if counted) */
   double  seconds_total;          /* wall time inside sc_rve_* calls */
+  int64_t fp32_evaluations;       /* residual evaluations with FP32 spectral stages (mixed precision) */
 } sc_stats;
 
 /* ------------------------------------------------------------------------- */

@@ -76,7 +76,7 @@ class Stats(C.Structure):
         ("residual_evaluations", C.c_int64), ("material_point_updates", C.c_int64),
         ("increments", C.c_int64), ("cutbacks", C.c_int64),
         ("newton_iterations", C.c_int64), ("state_iterations", C.c_int64),
-        ("seconds_total", C.c_double),
+        ("seconds_total", C.c_double), ("fp32_evaluations", C.c_int64),
     ]
 
 

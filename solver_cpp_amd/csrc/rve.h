@@ -106,6 +106,13 @@ class Rve {
   MatConst mc;
   MpState mp;
   FftPlan<double> fft;
+  // mixed precision: FP32 transforms and Gamma on an FP32 spectrum
+  bool mixed = false;
+  bool use32 = false;     // the current residual evaluations use the FP32 stages
+  int iter_base = 0;      // total_iter offset of an FP64 finishing solve
+  FftPlan<float> fft32;
+  float* spec32 = nullptr;
+  int64_t evals32 = 0, evals64 = 0;
 
   // device fields (SoA planes)
   double* X = nullptr;  // F_PETSc

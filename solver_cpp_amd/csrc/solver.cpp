@@ -294,6 +294,11 @@ Rve::Rve(const sc_rve_desc& d) {
   SCA_HIP(hipStreamCreate(&st));
   fft.init(cells);
   ngm = new Ngmres(9 * N, st, &kt);
+  mixed = d.precision == SC_PRECISION_MIXED;
+  if (mixed) {
+    fft32.init(cells);
+    SCA_HIP(hipMalloc(&spec32, 2 * 9 * K * sizeof(float)));
+  }
   // material point planes
   mp.N = N;
   mp.F = dalloc(9 * N);
@@ -397,6 +402,8 @@ Rve::Rve(const sc_rve_desc& d) {
 
 Rve::~Rve() {
   delete ngm;
+  if (spec32) (void)hipFree(spec32);
+  fft32.release();
   for (double* p : allocs) (void)hipFree(p);
   if (mp.iters) (void)hipFree(mp.iters);
   if (d_ill) (void)hipFree(d_ill);
@@ -512,8 +519,8 @@ GammaHost Rve::gamma_host(bool do_gamma) const {
 
 // MechSolverBasic::form_residual, mech_solver_basic.cpp:286-339
 void Rve::form_residual(const double* Fdev, double* rdev) {
-  if (nfuncs == 0 && snes_iter == 0) total_iter = -1;
-  if (total_iter <= snes_iter) total_iter += 1;
+  if (nfuncs == 0 && snes_iter == 0) total_iter = iter_base - 1;
+  if (total_iter - iter_base <= snes_iter) total_iter += 1;
   constitutive_response(Fdev, delta_t);
   // DC of the convolution: rotated S : P_av, computed on the device
   double R[9], Rrm[9];
@@ -521,16 +528,32 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Rrm[3 * i + j] = R[i + 3 * j];
   launch_bc_aim(small + SM_SUMS, S, Rrm, wgt, 1, small + SM_AIM, st);
-  kt.begin(SC_KT_FFT_FWD, st);
-  fft.forward(mp.P, spec, 9, st);
-  kt.end(SC_KT_FFT_FWD, N, st);
-  kt.begin(SC_KT_GAMMA, st);
-  launch_gamma_div<double>(spec, K, gamma_host(true), small + SM_AIM, d_xi_tab, partial, st);
-  launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
-  kt.end(SC_KT_GAMMA, K, st);
-  kt.begin(SC_KT_FFT_BWD, st);
-  fft.backward(spec, rdev, 9, 1.0, st);
-  kt.end(SC_KT_FFT_BWD, N, st);
+  if (use32) {  // FP32 transforms and spectrum; P in and r out stay FP64 planes
+    kt.begin(SC_KT_FFT_FWD, st);
+    fft32.forward(mp.P, spec32, 9, st);
+    kt.end(SC_KT_FFT_FWD, N, st);
+    kt.begin(SC_KT_GAMMA, st);
+    launch_gamma_div<float>(spec32, K, gamma_host(true), small + SM_AIM, d_xi_tab, partial, st);
+    launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
+    kt.end(SC_KT_GAMMA, K, st);
+    kt.begin(SC_KT_FFT_BWD, st);
+    fft32.backward(spec32, rdev, 9, 1.0, st);
+    kt.end(SC_KT_FFT_BWD, N, st);
+    evals32++;
+    stats.fp32_evaluations++;
+  } else {
+    kt.begin(SC_KT_FFT_FWD, st);
+    fft.forward(mp.P, spec, 9, st);
+    kt.end(SC_KT_FFT_FWD, N, st);
+    kt.begin(SC_KT_GAMMA, st);
+    launch_gamma_div<double>(spec, K, gamma_host(true), small + SM_AIM, d_xi_tab, partial, st);
+    launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
+    kt.end(SC_KT_GAMMA, K, st);
+    kt.begin(SC_KT_FFT_BWD, st);
+    fft.backward(spec, rdev, 9, 1.0, st);
+    kt.end(SC_KT_FFT_BWD, N, st);
+    evals64++;
+  }
   fetch_response_scalars(true);
   err_div = std::sqrt(last_div_sum) * wgt;
   if (cells[0] == 1) err_div = err_div / 2;
@@ -736,7 +759,6 @@ int Rve::snes_ngmres() {
   const int msize = desc.grid_num.ngmres_m > 0 ? desc.grid_num.ngmres_m : 30;
   snes_iter = 0;
   nfuncs = 0;
-  solve_evals = 0;
   return ngm->solve(
       X, msize, [this](const double* x, double* f) { form_residual(x, f); }, [this](int) { return converged(); },
       [this](int k) { snes_iter = k; });
@@ -818,7 +840,25 @@ sc_solution Rve::calculate_solution() {
   masked_compliance_host(C_volAvg, rot_q, stress_mask, S);
   if (desc.grid_num.update_gamma)
     for (int q = 0; q < 81; ++q) C_ref[q] = C_minMaxAvg[q] / wgt;
-  const int reason = snes_ngmres();
+  solve_evals = 0;
+  iter_base = 0;
+  int reason;
+  if (mixed) {
+    // FP32 spectral stages until the FP64-normed criteria hold, then an FP64
+    // solve continued from that F (its iterations count on from the first
+    // solve's), which accepts at its first evaluation when the FP64 residual
+    // already meets the tolerances
+    use32 = true;
+    reason = snes_ngmres();
+    use32 = false;
+    if (reason > 0 && !terminally_ill) {
+      iter_base = total_iter + 1;
+      reason = snes_ngmres();
+    }
+    iter_base = 0;
+  } else {
+    reason = snes_ngmres();
+  }
   tangent();
   sc_solution out{};
   out.converged = reason > 0;

@@ -18,9 +18,12 @@ struct FftPlan {
   T* tw[3] = {nullptr, nullptr, nullptr};  // forward twiddles per axis
   void init(const int cells[3]);
   void release();
-  // real: ncomp planes of N; spec: ncomp planes of K complex (interleaved re, im)
-  void forward(const T* real, T* spec, int ncomp, hipStream_t st) const;
-  void backward(T* spec, T* real, int ncomp, double scale, hipStream_t st) const;  // spec is clobbered
+  // real: ncomp planes of N (double or T, converted in the x pass); spec:
+  // ncomp planes of K complex T (interleaved re, im)
+  template <typename TI>
+  void forward(const TI* real, T* spec, int ncomp, hipStream_t st) const;
+  template <typename TO>
+  void backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st) const;  // spec is clobbered
   void strided_passes(void* spec, int ncomp, int sign, hipStream_t st) const;
 };
 

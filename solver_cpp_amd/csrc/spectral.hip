@@ -143,8 +143,8 @@ __device__ cpx<T>* stockham(cpx<T>* a, cpx<T>* b, int PB, int ps, const FftPassD
 
 // x pass, real -> half complex: rows R of Nx reals at in + R*Nx (one plane
 // of the field), output rows of nr complex at out + R*nr.
-template <typename T>
-__global__ void __launch_bounds__(256) fft_x_r2c(const T* in, cpx<T>* out, int64_t nrows, FftPassDesc d,
+template <typename T, typename TI>
+__global__ void __launch_bounds__(256) fft_x_r2c(const TI* in, cpx<T>* out, int64_t nrows, FftPassDesc d,
                                                  const cpx<T>* tw, int PB) {
   extern __shared__ unsigned char smem_raw[];
   const int n = d.n, nr = n / 2 + 1;
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) fft_x_r2c(const T* in, cpx<T>* out, int64
   for (int f = threadIdx.x; f < PB * n; f += blockDim.x) {
     const int j = f / n, e = f % n;
     const int64_t r = r0 + j;
-    a[f] = {(r < nrows) ? in[r * n + e] : (T)0, (T)0};
+    a[f] = {(r < nrows) ? (T)in[r * n + e] : (T)0, (T)0};
   }
   __syncthreads();
   cpx<T>* res = stockham(a, b, PB, n, d, tw);
@@ -166,8 +166,8 @@ __global__ void __launch_bounds__(256) fft_x_r2c(const T* in, cpx<T>* out, int64
 }
 
 // x pass, half complex -> real (Hermitian completion), scaled
-template <typename T>
-__global__ void __launch_bounds__(256) fft_x_c2r(const cpx<T>* in, T* out, int64_t nrows, FftPassDesc d,
+template <typename T, typename TO>
+__global__ void __launch_bounds__(256) fft_x_c2r(const cpx<T>* in, TO* out, int64_t nrows, FftPassDesc d,
                                                  const cpx<T>* tw, T scale, int PB) {
   extern __shared__ unsigned char smem_raw[];
   const int n = d.n, nr = n / 2 + 1;
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256) fft_x_c2r(const cpx<T>* in, T* out, int64
   for (int f = threadIdx.x; f < PB * n; f += blockDim.x) {
     const int j = f / n, e = f % n;
     const int64_t r = r0 + j;
-    if (r < nrows) out[r * n + e] = res[j * n + e].x * scale;
+    if (r < nrows) out[r * n + e] = (TO)(res[j * n + e].x * scale);
   }
 }
 
@@ -665,7 +665,8 @@ static int pencils_per_block(int n) {
 static int x_rows_per_block(int n) { return std::max(1, std::min(256, 2048 / n)); }
 
 template <typename T>
-void FftPlan<T>::forward(const T* real, T* spec, int ncomp, hipStream_t st) const {
+template <typename TI>
+void FftPlan<T>::forward(const TI* real, T* spec, int ncomp, hipStream_t st) const {
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
   // x: rows of all components
@@ -674,7 +675,7 @@ void FftPlan<T>::forward(const T* real, T* spec, int ncomp, hipStream_t st) cons
     const int PB = x_rows_per_block(Nx);
     const int64_t rows = (int64_t)ncomp * Ny * Nz;
     const size_t sh = 2 * (size_t)PB * Nx * sizeof(cpx<T>);
-    hipLaunchKernelGGL(fft_x_r2c<T>, dim3(nblk(rows, PB)), dim3(256), sh, st, real, S, rows, d,
+    hipLaunchKernelGGL((fft_x_r2c<T, TI>), dim3(nblk(rows, PB)), dim3(256), sh, st, real, S, rows, d,
                        reinterpret_cast<const cpx<T>*>(tw[0]), PB);
     SCA_HIP(hipGetLastError());
   }
@@ -704,7 +705,8 @@ void FftPlan<T>::strided_passes(void* Sv, int ncomp, int sign, hipStream_t st) c
 }
 
 template <typename T>
-void FftPlan<T>::backward(T* spec, T* real, int ncomp, double scale, hipStream_t st) const {
+template <typename TO>
+void FftPlan<T>::backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st) const {
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
   strided_passes(S, ncomp, +1, st);
@@ -712,13 +714,19 @@ void FftPlan<T>::backward(T* spec, T* real, int ncomp, double scale, hipStream_t
   const int PB = x_rows_per_block(Nx);
   const int64_t rows = (int64_t)ncomp * Ny * Nz;
   const size_t sh = 2 * (size_t)PB * Nx * sizeof(cpx<T>);
-  hipLaunchKernelGGL(fft_x_c2r<T>, dim3(nblk(rows, PB)), dim3(256), sh, st, S, real, rows, d,
+  hipLaunchKernelGGL((fft_x_c2r<T, TO>), dim3(nblk(rows, PB)), dim3(256), sh, st, S, real, rows, d,
                      reinterpret_cast<const cpx<T>*>(tw[0]), (T)scale, PB);
   SCA_HIP(hipGetLastError());
 }
 
 template struct FftPlan<double>;
 template struct FftPlan<float>;
+template void FftPlan<double>::forward<double>(const double*, double*, int, hipStream_t) const;
+template void FftPlan<double>::backward<double>(double*, double*, int, double, hipStream_t) const;
+template void FftPlan<float>::forward<double>(const double*, float*, int, hipStream_t) const;
+template void FftPlan<float>::backward<double>(float*, double*, int, double, hipStream_t) const;
+template void FftPlan<float>::forward<float>(const float*, float*, int, hipStream_t) const;
+template void FftPlan<float>::backward<float>(float*, float*, int, double, hipStream_t) const;
 
 static GammaArgs gamma_args(const GammaHost& gh, const double* d_xi_tab, const double* d_aim) {
   GammaArgs g;

@@ -252,3 +252,27 @@ def test_loadcase_16cubed_20grains(product, oracle):
     close(a.field("xi"), b.field("xi"), REL, "xi")
     close(a.scalars().P_av, b.scalars().P_av, REL, "P_av")
     close(a.field("x_p"), b.field("x_p"), 1e-12, "x_p")
+
+
+def test_mixed_precision_16cubed(product, oracle):
+    """Mixed form on the CPU-runnable configuration: FP32 transforms and Gamma
+    (FP64 material point, FP64 norms) until the convergence criteria hold,
+    then an FP64 solve continued from that F, against the FP64 oracle.  Both
+    results satisfy the solver tolerances; they differ by a fraction of them:
+    <P> within 2e-5 relative and every cell's P within 1e-4 of max |P|."""
+    cfg = CONFIGS["C1"]
+    e = cfg["edge"]
+    mat, ori, size = polycrystal((e, e, e), cfg["grains"], seed=cfg["seed"], ori_seed=cfg["ori_seed"])
+    a = Rve(product, (e, e, e), size, mat, ori, precision=1, **EXAMPLE_NUMERICS)
+    b = Rve(oracle, (e, e, e), size, mat, ori, **EXAMPLE_NUMERICS)
+    a.init()
+    b.init()
+    load = [tension_x(t=cfg["t"], N=cfg["N"])]
+    ia, ea = a.loop_over_loadcase(load)
+    ib, eb = b.loop_over_loadcase(load)
+    st = a.stats()
+    assert st.fp32_evaluations > 0
+    assert st.residual_evaluations - st.fp32_evaluations >= len(ia)  # every increment ends in FP64
+    pa, pb = np.array(a.scalars().P_av), np.array(b.scalars().P_av)
+    close(pa, pb, 2e-5, "<P>")
+    close(a.field("P"), b.field("P"), 1e-4, "P")
