@@ -198,7 +198,8 @@ enum {
   SC_FIELD_XI = 7,       /* (12,N) slip resistance */
   SC_FIELD_GAMMA = 8,    /* (12,N) accumulated shear */
   SC_FIELD_X_P = 9,      /* (3,N) cell centre coordinates (after update_coords) */
-  SC_FIELD_X_N = 10      /* (3,(Nx+1)(Ny+1)(Nz+1)) node coordinates */
+  SC_FIELD_X_N = 10,     /* (3,(Nx+1)(Ny+1)(Nz+1)) node coordinates */
+  SC_FIELD_ITERATIONS = 11 /* (2,N) Lp Newton and state iterations of the last evaluation, as doubles */
 };
 int sc_rve_get_field(sc_rve* rve, int field, double* out, int64_t n_values);
 int sc_rve_set_field(sc_rve* rve, int field, const double* in, int64_t n_values);
@@ -236,6 +237,46 @@ typedef struct sc_kernel_times {
 } sc_kernel_times;
 int sc_rve_profile(sc_rve* rve, int enable);
 int sc_rve_get_kernel_times(sc_rve* rve, sc_kernel_times* out);
+
+/* ------------------------------------------------------------------------- */
+/* Job server: continuous batching of RVE load increments on one GPU          */
+/* (no counterpart in the reference: its driver runs one RVE per process).   */
+/* ------------------------------------------------------------------------- */
+
+/* Worst-case device bytes of one RVE context (admission footprint). */
+int64_t sc_rve_footprint(const sc_rve_desc* desc);
+
+typedef struct sc_server sc_server;
+typedef struct sc_request_result {
+  int64_t request, job;
+  int     status;                 /* 0 solved, 1 load case exhausted, 2 not solved, 3 error */
+  int     increment;              /* the job's increment count after this request */
+  int     iterations, evaluations;
+  double  P_av[9];                /* volume-averaged P after the increment */
+  double  t;                      /* load-case time after the increment, s */
+  int64_t material_point_updates; /* cells x residual evaluations */
+  double  t_submit, t_start, t_end; /* seconds since sc_server_create */
+} sc_request_result;
+typedef struct sc_server_stats {
+  int64_t budget_bytes, used_bytes, peak_bytes;
+  int64_t resident_jobs, completed_requests;
+  int     max_concurrent;         /* most requests running at once */
+} sc_server_stats;
+
+/* workers: host threads, each driving one admitted job at a time on its own
+ * HIP stream; hbm_budget_bytes bounds the sum of resident jobs' footprints. */
+int  sc_server_create(int64_t hbm_budget_bytes, int workers, sc_server** out);
+void sc_server_destroy(sc_server* server);  /* finishes running requests */
+/* A job: RVE description (copied) and its load case.  keep_resident = 0
+ * releases the job's memory as soon as it has no pending request. */
+int  sc_server_add_job(sc_server* server, const sc_rve_desc* desc, const sc_loadstep* steps, int n_steps,
+                       int keep_resident, int64_t* job);
+/* One increment of the job's load case; increments of a job run in order. */
+int  sc_server_submit(sc_server* server, int64_t job, int64_t* request);
+/* Blocks until done (timeout_s < 0: no limit; returns 4 on timeout). */
+int  sc_server_wait(sc_server* server, int64_t request, double timeout_s, sc_request_result* out);
+int  sc_server_remove_job(sc_server* server, int64_t job);
+int  sc_server_get_stats(sc_server* server, sc_server_stats* out);
 
 /* ------------------------------------------------------------------------- */
 /* Spectral building blocks (parity tests against the reference's goldens).  */

@@ -173,8 +173,9 @@ void form_residual(sc_rve* R, const double* F, double* r) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) e = std::max(e, std::fabs(m.a[i][j]));
   R->err_BC = e;
-  // rotate_tensor2(.., true) passes -1, which rotations.f90:359 treats as passive
-  M3 rotated = rotTensor2(R->rot_q, dFaim, false);
+  // load frame -> field frame: the active rotation mech_solver_basic.cpp:333
+  // asks for (its -1 flag reaches rotations.f90:359 as passive; see DESIGN)
+  M3 rotated = rotTensor2(R->rot_q, dFaim, true);
   gamma_convolution(R, r, rotated);
   R->nfuncs++;
   R->solve_evals++;
@@ -605,7 +606,7 @@ int sc_rve_forward(sc_rve* R, int cut_back, int guess, double dt, double dt_old,
           for (int j = 0; j < 3; ++j) step.a[i][j] = (vals.a[i][j] - R->F_aim_last_inc.a[i][j]) / t_remaining;
       }
       R->F_aim_dot = add(R->F_aim_dot, maskmerge0(step, st->deformation_mask));
-      M3 rotated = rotTensor2(st->rot_q, R->F_aim_dot, false);
+      M3 rotated = rotTensor2(st->rot_q, R->F_aim_dot, true);  // load -> field frame, active
       std::vector<double> F0 = R->F;
       calculate_rate(N, guess != 0, R->F_last_inc.data(), F0.data(), dt_old, rotated, R->F_dot.data());
       R->F_last_inc = F0;
@@ -624,7 +625,7 @@ int sc_rve_forward(sc_rve* R, int cut_back, int guess, double dt, double dt_old,
           R->P_aim.a[i][j] += st->stress[q] * dt;
       }
     }
-    M3 rotated = rotTensor2(st->rot_q, R->F_aim, false);
+    M3 rotated = rotTensor2(st->rot_q, R->F_aim, true);  // load -> field frame, active
     std::vector<double> Fn(9 * N);
     forward_field(N, R->g.wgt, dt, R->F_last_inc.data(), R->F_dot.data(), &rotated, Fn.data());
     R->F = Fn;
@@ -803,6 +804,13 @@ int sc_rve_get_field(sc_rve* R, int field, double* out, int64_t n) {
         need(12 * N);
         for (int64_t c = 0; c < N; ++c)
           for (int i = 0; i < 12; ++i) out[12 * c + i] = field == SC_FIELD_XI ? R->pts[c].xi[i] : R->pts[c].gamma[i];
+        break;
+      case SC_FIELD_ITERATIONS:
+        need(2 * N);
+        for (int64_t c = 0; c < N; ++c) {
+          out[2 * c] = R->pts[c].newton_iter;
+          out[2 * c + 1] = R->pts[c].state_iter;
+        }
         break;
       case SC_FIELD_X_P:
         need((int64_t)R->x_p.size());

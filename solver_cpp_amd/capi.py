@@ -90,6 +90,22 @@ class SolverScalars(C.Structure):
     ]
 
 
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("request", C.c_int64), ("job", C.c_int64), ("status", C.c_int), ("increment", C.c_int),
+        ("iterations", C.c_int), ("evaluations", C.c_int), ("P_av", C.c_double * 9), ("t", C.c_double),
+        ("material_point_updates", C.c_int64),
+        ("t_submit", C.c_double), ("t_start", C.c_double), ("t_end", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64), ("peak_bytes", C.c_int64),
+        ("resident_jobs", C.c_int64), ("completed_requests", C.c_int64), ("max_concurrent", C.c_int),
+    ]
+
+
 KT_CLASSES = ["response", "tangent", "fft_fwd", "fft_bwd", "gamma", "reduce", "other"]
 
 
@@ -97,7 +113,7 @@ class KernelTimes(C.Structure):
     _fields_ = [("launches", C.c_int64 * 7), ("ms", C.c_double * 7), ("points", C.c_int64 * 7)]
 
 
-SC_FIELD = dict(F=0, P=1, Fp=2, Lp=3, S=4, Fe=5, F_last_inc=6, xi=7, gamma=8, x_p=9, x_n=10)
+SC_FIELD = dict(F=0, P=1, Fp=2, Lp=3, S=4, Fe=5, F_last_inc=6, xi=7, gamma=8, x_p=9, x_n=10, iterations=11)
 SC_DEF = dict(F=0, dot_F=1, L=2)
 SC_STRESS = dict(P=0, dot_P=1)
 PRECISION = dict(fp64=0, mixed=1)
@@ -152,7 +168,7 @@ PROFILE_SIGNATURES = {
 }
 
 # Symbols only the product exports: the drop-in boundary of the Fortran core
-# (f_*) and the job-input readers.
+# (f_*), the job-input readers and the job server.
 BOUNDARY_SIGNATURES = {
     "f_materialpoint_initBase": (None, [C.c_char_p, C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
                                         C.c_char_p, C.c_int]),
@@ -174,6 +190,15 @@ BOUNDARY_SIGNATURES = {
     "sc_read_material": (C.c_int, [C.c_char_p, C.POINTER(Material), _dp, C.c_int, _ip]),
     "sc_read_numerics": (C.c_int, [C.c_char_p, C.POINTER(GridNumerics), C.POINTER(PhaseNumerics)]),
     "sc_read_load": (C.c_int, [C.c_char_p, C.POINTER(LoadStep), C.c_int, _ip]),
+    "sc_rve_footprint": (C.c_int64, [C.POINTER(RveDesc)]),
+    "sc_server_create": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),
+    "sc_server_destroy": (None, [C.c_void_p]),
+    "sc_server_add_job": (C.c_int, [C.c_void_p, C.POINTER(RveDesc), C.POINTER(LoadStep), C.c_int, C.c_int,
+                                    C.POINTER(C.c_int64)]),
+    "sc_server_submit": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "sc_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(RequestResult)]),
+    "sc_server_remove_job": (C.c_int, [C.c_void_p, C.c_int64]),
+    "sc_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
 }
 
 

@@ -331,6 +331,17 @@ int sc_rve_get_field(sc_rve* R, int field, double* out, int64_t n) {
       case SC_FIELD_F_LAST_INC: src = v.F_last_inc; break;
       case SC_FIELD_XI: src = v.mp.xi; ncomp = 12; break;
       case SC_FIELD_GAMMA: src = v.mp.gamma; ncomp = 12; break;
+      case SC_FIELD_ITERATIONS: {
+        if (n < 2 * N) throw std::runtime_error("output buffer too small");
+        std::vector<int> it(2 * N);
+        SCA_HIP(hipMemcpyAsync(it.data(), v.mp.iters, 2 * N * sizeof(int), hipMemcpyDeviceToHost, v.st));
+        SCA_HIP(hipStreamSynchronize(v.st));
+        for (int64_t c = 0; c < N; ++c) {
+          out[2 * c] = it[c];
+          out[2 * c + 1] = it[N + c];
+        }
+        return 0;
+      }
       case SC_FIELD_X_P:
       case SC_FIELD_X_N: {
         const int64_t m = field == SC_FIELD_X_P ? 3 * N

@@ -45,8 +45,12 @@ __device__ constexpr int HTYPE[NSLIP][NSLIP] = {
 __device__ __forceinline__ constexpr int kcoefA(int s, int m, int l) { return psg(s, m, l) + psg(s, l, m); }
 __device__ __forceinline__ constexpr int kcoefB(int s, int m, int l) { return m == l ? psg(s, m, m) : 0; }
 
+// Material constants travel in the kernel arguments (not a __constant__
+// symbol) so that launches of different contexts on concurrent streams never
+// share them.
 struct KernelArgs {
   MpState s;
+  MatConst mc;
   double dt;
   int* ill;
   int* broken;  // per point, may be null
@@ -54,7 +58,6 @@ struct KernelArgs {
   int64_t c_begin, c_end;  // cells [c_begin, c_end) of the N-cell planes
 };
 
-__constant__ MatConst g_mc;
 
 // ---- small dense helpers (row-major 3x3 in registers: m[3*i+j]) ----------
 __device__ __forceinline__ void mm33(const double* A, const double* B, double* C) {
@@ -471,7 +474,7 @@ constexpr int SUB_F0 = 0, SUB_LP0 = 9, SUB_FP0 = 18, SUB_ST0 = 27, SUB_FLOC = 51
 // One residual evaluation's material-point loop.
 template <int NE4, int AE4>
 __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
-  const MatConst& mc = g_mc;
+  const MatConst& mc = a.mc;
   const MpState& s = a.s;
   const int64_t N = s.N;
   const int64_t c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -675,6 +678,7 @@ __global__ void __launch_bounds__(256) pav_partial_kernel(const double* P, int64
 namespace {
 struct TangentArgs {
   MpState s;
+  MatConst mc;
   double dt;
   double* dPdF;
   double C[81];  // C(i,j,k,l) at 27i+9j+3k+l (row-major)
@@ -687,7 +691,7 @@ __device__ __forceinline__ double Cijkl(const TangentArgs& a, int i, int j, int 
 }  // namespace
 
 __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
-  const MatConst& mc = g_mc;
+  const MatConst& mc = a.mc;
   const MpState& s = a.s;
   const int64_t N = s.N;
   const int64_t c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -968,8 +972,7 @@ void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_
   if (c_end < 0) c_end = s.N;
   if (c_begin < 0 || c_end > s.N || c_begin > c_end) throw std::runtime_error("material-point range out of bounds");
   if (c_begin == c_end) return;
-  SCA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mc), &mc, sizeof(MatConst), 0, hipMemcpyHostToDevice, st));
-  KernelArgs a{s, dt, d_ill, d_broken, d_sub, c_begin, c_end};
+  KernelArgs a{s, mc, dt, d_ill, d_broken, d_sub, c_begin, c_end};
   const int blocks = (int)((c_end - c_begin + 63) / 64);
   // exponents of the example material (n_sl = 20, a_sl = 2.25) are compiled
   // in; any other material runs the runtime-exponent instantiation
@@ -990,9 +993,9 @@ void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* 
   if (c_end < 0) c_end = s.N;
   if (c_begin < 0 || c_end > s.N || c_begin > c_end) throw std::runtime_error("material-point range out of bounds");
   if (c_begin == c_end) return;
-  SCA_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(g_mc), &mc, sizeof(MatConst), 0, hipMemcpyHostToDevice, st));
   TangentArgs a;
   a.s = s;
+  a.mc = mc;
   a.dt = dt;
   a.dPdF = dPdF;
   a.c_begin = c_begin;

@@ -76,6 +76,13 @@ class Ngmres {
   double* slot(int k);
 };
 
+// position in a load case (the loop variables of loop_over_loadcase)
+struct LoadCursor {
+  int l = 0, inc = 0, total_inc = 0, cut_back_level = 0;
+  double t = 0.0, t_0 = 0.0, dt = 1.0, dt_prev = 0.0;
+  bool cut_back = false, guess = false, step_started = false;
+};
+
 class Rve {
  public:
   explicit Rve(const sc_rve_desc& d);
@@ -87,6 +94,7 @@ class Rve {
   void forward(bool cutBack, bool guess, double dt, double dt_old, double t_remaining, const sc_loadstep& stp);
   sc_solution calculate_solution();
   int run_loadcase(const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations, int max_incs);
+  int next_increment(const sc_loadstep* steps, int n_steps, LoadCursor& cur, int* iterations, int* evaluations);
   void form_residual(const double* Fdev, double* rdev);
   void constitutive_response(const double* Fdev, double dt);
   void fetch_response_scalars(bool rotate);

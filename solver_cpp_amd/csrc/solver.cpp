@@ -51,6 +51,14 @@ void rot2(const double q[4], const double* T, double* out) {
   mul(R, T, t);
   mul(t, Rt, out);
 }
+// active R^T T R: load frame -> field frame
+void rot2_active(const double q[4], const double* T, double* out) {
+  double R[9], Rt[9], t[9];
+  qu2om_host(q, R);
+  tr(R, Rt);
+  mul(Rt, T, t);
+  mul(t, R, out);
+}
 // rotate_tensor4, rotations.f90:308-328 (passive)
 void rot4(const double q[4], const double* T, double* out) {
   double R[9];
@@ -291,7 +299,9 @@ Rve::Rve(const sc_rve_desc& d) {
     }
   }
   build_matconst(d.material, d.phase_num, mc);
-  SCA_HIP(hipStreamCreate(&st));
+  // non-blocking: contexts driven from several host threads never wait on
+  // each other through the null stream
+  SCA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   fft.init(cells);
   ngm = new Ngmres(9 * N, st, &kt);
   mixed = d.precision == SC_PRECISION_MIXED;
@@ -365,7 +375,8 @@ Rve::Rve(const sc_rve_desc& d) {
           }
         }
     d_xi_tab = dalloc(6 * K);
-    SCA_HIP(hipMemcpy(d_xi_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    SCA_HIP(hipMemcpyAsync(d_xi_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    SCA_HIP(hipStreamSynchronize(st));
   }
   // mechanical_init: F = I, Fp = O_0, Fe = Fp^T, Lp = S = 0, xi = xi_0, gamma = 0
   {
@@ -378,7 +389,7 @@ Rve::Rve(const sc_rve_desc& d) {
       for (int q = 0; q < 9; ++q) fp[q * N + c] = om[9 * ma + q];
     }
     double* d_fp = dalloc(9 * N);
-    SCA_HIP(hipMemcpy(d_fp, fp.data(), fp.size() * sizeof(double), hipMemcpyHostToDevice));
+    SCA_HIP(hipMemcpyAsync(d_fp, fp.data(), fp.size() * sizeof(double), hipMemcpyHostToDevice, st));
     launch_mp_init(mp, d_fp, d.material.xi_0_sl, st);
     SCA_HIP(hipStreamSynchronize(st));
     SCA_HIP(hipFree(d_fp));
@@ -462,7 +473,8 @@ void Rve::fetch_response_scalars(bool rotate) {
   if (ill) terminally_ill = true;
   if (kt.on) {
     double it[2];
-    SCA_HIP(hipMemcpy(it, small + SM_ITERS, sizeof it, hipMemcpyDeviceToHost));
+    SCA_HIP(hipMemcpyAsync(it, small + SM_ITERS, sizeof it, hipMemcpyDeviceToHost, st));
+    SCA_HIP(hipStreamSynchronize(st));
     stats.newton_iterations += (int64_t)it[0];
     stats.state_iterations += (int64_t)it[1];
   }
@@ -779,8 +791,9 @@ void Rve::init() {
   std::vector<double> eye(9 * N);
   for (int q = 0; q < 9; ++q)
     for (int64_t c = 0; c < N; ++c) eye[q * N + c] = (q % 4 == 0) ? 1.0 : 0.0;
-  SCA_HIP(hipMemcpy(F_last_inc, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice));
-  SCA_HIP(hipMemcpy(X, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice));
+  SCA_HIP(hipMemcpyAsync(F_last_inc, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice, st));
+  SCA_HIP(hipMemcpyAsync(X, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice, st));
+  SCA_HIP(hipStreamSynchronize(st));
   update_coords();
   constitutive_response(X, 0.0);
   fetch_response_scalars(false);
@@ -805,7 +818,7 @@ void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_r
                                                         : (stp.deformation[q] - F_aim_last_inc[q]) / t_remaining;
     for (int q = 0; q < 9; ++q) F_aim_dot[q] += stp.deformation_mask[q] ? 0.0 : step[q];
     double rotated[9];
-    h::rot2(stp.rot_q, F_aim_dot, rotated);
+    h::rot2_active(stp.rot_q, F_aim_dot, rotated);  // load -> field frame
     SCA_HIP(hipMemcpyAsync(small + SM_AV, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
     launch_rate(guess ? 1 : 0, F_last_inc, X, dt_old, small + SM_AV, F_dot, N, st);
     SCA_HIP(hipMemcpyAsync(F_last_inc, X, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -820,7 +833,7 @@ void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_r
         P_aim[q] += stp.stress[q] * dt;
     }
   double rotated[9];
-  h::rot2(stp.rot_q, F_aim, rotated);
+  h::rot2_active(stp.rot_q, F_aim, rotated);  // load -> field frame
   SCA_HIP(hipMemcpyAsync(small + SM_AIM2, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
   launch_fwd_field(F_last_inc, F_dot, dt, X, 9 * N, st);
   std::vector<int> ops(9, 2);
@@ -874,58 +887,78 @@ sc_solution Rve::calculate_solution() {
   return out;
 }
 
-int Rve::run_loadcase(const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations,
-                      int max_incs) {
+// One increment of DamaskGrid::loop_over_loadcase (damask_grid.cpp:82-196)
+// at the cursor: time step, forward, solution and cut-backs.  Returns 0 when
+// an increment was solved, 1 when the load case is exhausted, 2 when the
+// increment could not be solved within max_cut_back.
+int Rve::next_increment(const sc_loadstep* steps, int n_steps, LoadCursor& cur, int* iterations, int* evaluations) {
   const auto& gn = desc.grid_num;
-  bool cut_back = false;
   const int sub_step_factor = 2;
-  int cut_back_level = 0, total_inc = 0;
-  double t = 0, dt = 1, dt_prev = 0, t_remaining = 0;
-  for (int l = 0; l < n_steps; ++l) {
-    const sc_loadstep& stp = steps[l];
-    const double t_0 = t;
-    bool guess = stp.estimate_rate && l > 0;
-    for (int inc = 0; inc < stp.N; ++inc) {
-      total_inc++;
-      dt_prev = dt;
-      if (std::fabs(stp.r - 1.0) < 1.e-9)
-        dt = stp.t / stp.N;
-      else
-        dt = stp.t * (std::pow(stp.r, inc) - std::pow(stp.r, inc + 1)) / (1.0 - std::pow(stp.r, stp.N));
-      dt *= std::pow((double)sub_step_factor, -(double)cut_back_level);
-      int step_fraction = 0, iters = 0, evals = 0;
-      while (step_fraction < std::pow(sub_step_factor, cut_back_level)) {
-        t_remaining = stp.t + t_0 - t;
-        t += dt;
-        step_fraction++;
-        forward(cut_back, guess, dt, dt_prev, t_remaining, stp);
-        const sc_solution sol = calculate_solution();
-        iters += sol.required_iterations;
-        evals += sol.residual_evaluations;
-        if (sol.converged && sol.stag_converged && !sol.terminally_ill) {
-          update_coords();
-          dt_prev = dt;
-          cut_back = false;
-          guess = true;
-        } else if (cut_back_level < gn.max_cut_back) {
-          cut_back = true;
-          step_fraction = (step_fraction - 1) * sub_step_factor;
-          cut_back_level++;
-          t = t - dt;
-          dt = dt / sub_step_factor;
-          stats.cutbacks++;
-        } else {
-          g_err = "no solution found for increment " + std::to_string(total_inc);
-          return 2;
-        }
-      }
-      cut_back_level = std::max(0, (int)cut_back - 1);  // damask_grid.cpp:153
-      stats.increments++;
-      if (inc_iterations && total_inc <= max_incs) inc_iterations[total_inc - 1] = iters;
-      if (inc_evaluations && total_inc <= max_incs) inc_evaluations[total_inc - 1] = evals;
+  if (cur.l >= n_steps) return 1;
+  const sc_loadstep& stp = steps[cur.l];
+  if (!cur.step_started) {
+    cur.t_0 = cur.t;
+    cur.guess = stp.estimate_rate && cur.l > 0;
+    cur.step_started = true;
+    cur.inc = 0;
+  }
+  const int inc = cur.inc;
+  cur.total_inc++;
+  cur.dt_prev = cur.dt;
+  if (std::fabs(stp.r - 1.0) < 1.e-9)
+    cur.dt = stp.t / stp.N;
+  else
+    cur.dt = stp.t * (std::pow(stp.r, inc) - std::pow(stp.r, inc + 1)) / (1.0 - std::pow(stp.r, stp.N));
+  cur.dt *= std::pow((double)sub_step_factor, -(double)cur.cut_back_level);
+  int step_fraction = 0, iters = 0, evals = 0;
+  while (step_fraction < std::pow(sub_step_factor, cur.cut_back_level)) {
+    const double t_remaining = stp.t + cur.t_0 - cur.t;
+    cur.t += cur.dt;
+    step_fraction++;
+    forward(cur.cut_back, cur.guess, cur.dt, cur.dt_prev, t_remaining, stp);
+    const sc_solution sol = calculate_solution();
+    iters += sol.required_iterations;
+    evals += sol.residual_evaluations;
+    if (sol.converged && sol.stag_converged && !sol.terminally_ill) {
+      update_coords();
+      cur.dt_prev = cur.dt;
+      cur.cut_back = false;
+      cur.guess = true;
+    } else if (cur.cut_back_level < gn.max_cut_back) {
+      cur.cut_back = true;
+      step_fraction = (step_fraction - 1) * sub_step_factor;
+      cur.cut_back_level++;
+      cur.t = cur.t - cur.dt;
+      cur.dt = cur.dt / sub_step_factor;
+      stats.cutbacks++;
+    } else {
+      g_err = "no solution found for increment " + std::to_string(cur.total_inc);
+      return 2;
     }
   }
+  cur.cut_back_level = std::max(0, (int)cur.cut_back - 1);  // damask_grid.cpp:153
+  stats.increments++;
+  if (iterations) *iterations = iters;
+  if (evaluations) *evaluations = evals;
+  cur.inc++;
+  if (cur.inc >= stp.N) {
+    cur.l++;
+    cur.step_started = false;
+  }
   return 0;
+}
+
+int Rve::run_loadcase(const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations,
+                      int max_incs) {
+  LoadCursor cur;
+  for (;;) {
+    int iters = 0, evals = 0;
+    const int rc = next_increment(steps, n_steps, cur, &iters, &evals);
+    if (rc == 1) return 0;
+    if (rc != 0) return rc;
+    if (inc_iterations && cur.total_inc <= max_incs) inc_iterations[cur.total_inc - 1] = iters;
+    if (inc_evaluations && cur.total_inc <= max_incs) inc_evaluations[cur.total_inc - 1] = evals;
+  }
 }
 
 }  // namespace sca

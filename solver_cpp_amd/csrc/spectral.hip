@@ -643,7 +643,8 @@ void FftPlan<T>::init(const int cells_[3]) {
       h[2 * t + 1] = (T)-std::sin(ang);
     }
     SCA_HIP(hipMalloc(&tw[p], 2 * n * sizeof(T)));
-    SCA_HIP(hipMemcpy(tw[p], h.data(), 2 * n * sizeof(T), hipMemcpyHostToDevice));
+    SCA_HIP(hipMemcpyAsync(tw[p], h.data(), 2 * n * sizeof(T), hipMemcpyHostToDevice, hipStreamPerThread));
+    SCA_HIP(hipStreamSynchronize(hipStreamPerThread));
     make_desc(n, -1);  // validate
   }
 }
@@ -912,7 +913,7 @@ void launch_gather81(const double* d, int64_t N, int64_t c, double* out, hipStre
   SCA_HIP(hipGetLastError());
 }
 // dF_aim on the device: P_av = wgt * sums (rotated R P R^T when rotating),
-// dF = S : P_av, aim = R dF R^T (the passive rotation of rotate_tensor2)
+// dF = S : P_av in the load frame, aim = R^T dF R in the field frame (active)
 struct BcArgs {
   double S[81];  // (i,j,k,l) column-major
   double R[9];   // row-major
@@ -941,10 +942,10 @@ __global__ void bc_aim_kernel(const double* sums, BcArgs b, double* aim) {
       dF[i][j] = s;
     }
   for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) t[i][j] = b.R[3 * i] * dF[0][j] + b.R[3 * i + 1] * dF[1][j] + b.R[3 * i + 2] * dF[2][j];
+    for (int j = 0; j < 3; ++j) t[i][j] = b.R[i] * dF[0][j] + b.R[3 + i] * dF[1][j] + b.R[6 + i] * dF[2][j];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
-      aim[i + 3 * j] = t[i][0] * b.R[3 * j] + t[i][1] * b.R[3 * j + 1] + t[i][2] * b.R[3 * j + 2];
+      aim[i + 3 * j] = t[i][0] * b.R[j] + t[i][1] * b.R[3 + j] + t[i][2] * b.R[6 + j];
 }
 void launch_bc_aim(const double* sums, const double S[81], const double R[9], double wgt, int rotate, double* aim,
                    hipStream_t st) {

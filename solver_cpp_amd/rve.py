@@ -62,13 +62,12 @@ def tension_x(t: float = 5.0, N: int = 10, rate: float = 1.0e-3, rot_q=(1.0, 0.0
                 stress=[["x", "x", "x"], ["x", 0, "x"], ["x", "x", 0]], t=t, N=N, rot_q=rot_q)
 
 
-class Rve:
-    """One representative volume element on one library backend."""
+class RveSpec:
+    """An sc_rve_desc together with the arrays it points to (kept alive here)."""
 
     def __init__(self, lib: Library, cells, size, material_at: np.ndarray, orientations: np.ndarray,
                  material: Optional[Material] = None, phase_num: Optional[PhaseNumerics] = None,
                  grid_num: Optional[GridNumerics] = None, precision: int = 0, **grid_overrides):
-        self.lib = lib
         m, p, g = lib.defaults()
         if material is not None:
             m = material
@@ -93,8 +92,20 @@ class Rve:
         d.material, d.phase_num, d.grid_num = m, p, g
         d.precision = int(precision)
         self.desc = d
+
+
+class Rve:
+    """One representative volume element on one library backend."""
+
+    def __init__(self, lib: Library, cells, size, material_at: np.ndarray, orientations: np.ndarray,
+                 material: Optional[Material] = None, phase_num: Optional[PhaseNumerics] = None,
+                 grid_num: Optional[GridNumerics] = None, precision: int = 0, **grid_overrides):
+        self.lib = lib
+        self.spec = RveSpec(lib, cells, size, material_at, orientations, material, phase_num, grid_num, precision,
+                            **grid_overrides)
+        self.cells, self.N, self.desc = self.spec.cells, self.spec.N, self.spec.desc
         h = C.c_void_p()
-        lib.check(lib.sc_rve_create(C.byref(d), C.byref(h)), "sc_rve_create")
+        lib.check(lib.sc_rve_create(C.byref(self.desc), C.byref(h)), "sc_rve_create")
         self.h = h
 
     def close(self):
@@ -144,7 +155,7 @@ class Rve:
 
     # access -------------------------------------------------------------------
     def field(self, name: str) -> np.ndarray:
-        n = {"xi": 12, "gamma": 12, "x_p": 3}.get(name, 9) * self.N
+        n = {"xi": 12, "gamma": 12, "x_p": 3, "iterations": 2}.get(name, 9) * self.N
         if name == "x_n":
             n = 3 * (self.cells[0] + 1) * (self.cells[1] + 1) * (self.cells[2] + 1)
         out = np.zeros(n)

@@ -375,3 +375,45 @@ def test_mp_tangent_matches_finite_differences(lib):
     # the analytic tangent agrees to ~3e-6 (plateau in h); a transposed or
     # mis-signed term gives O(1) differences
     assert np.abs(d - fd).max() < 1e-5 * np.abs(d).max()
+
+
+def _om(lib, q):
+    om = np.zeros(9)
+    lib.sc_qu2om(ptr(np.array(q, dtype=float)), ptr(om))
+    return om.reshape(3, 3, order="F")
+
+
+def _qu_of(lib, O):
+    """A unit quaternion q with qu2om(q) == O (tries q and its conjugate)."""
+    from scipy.spatial.transform import Rotation
+    x, y, z, w = Rotation.from_matrix(O).as_quat()
+    for q in ([w, x, y, z], [w, -x, -y, -z]):
+        q = np.array(q) * (1 if q[0] >= 0 else -1)
+        if np.allclose(_om(lib, q), O, atol=1e-13):
+            return q
+    raise AssertionError("no quaternion reproduces the matrix")
+
+
+def test_rotated_load_frame(lib):
+    """A load step with rotation R on a single crystal of orientation O is the
+    same problem as the unrotated load on the crystal O R^T (the crystal seen
+    from the load frame): load-frame P_av and iteration counts agree.  This
+    fixes the directions of the frame changes: field values enter the load
+    frame passively (R P R^T), load-frame aims enter the field actively
+    (R^T F R)."""
+    q0 = np.array(Q2)
+    qR = np.array([0.9, 0.3, -0.2, 0.25])
+    qR /= np.linalg.norm(qR)
+    O, Rm = _om(lib, q0), _om(lib, qR)
+    q1 = _qu_of(lib, O @ Rm.T)
+    mat = np.zeros(8, dtype=np.int32)
+    out = []
+    for q_crystal, q_load in ((q0, qR), (q1, (1.0, 0.0, 0.0, 0.0))):
+        R = Rve(lib, (2, 2, 2), (1e-5, 1e-5, 1e-5), mat, np.array([q_crystal]))
+        R.init()
+        its, evs = R.loop_over_loadcase([tension_x(t=2.0, N=2, rot_q=q_load)])
+        out.append((its, np.array(R.scalars().P_av)))
+    (i0, p0), (i1, p1) = out
+    np.testing.assert_array_equal(i0, i1)
+    assert np.abs(p0 - p1).max() < 1e-8 * np.abs(p1).max()
+    assert p0[0] > 1e7 and max(abs(p0[4]), abs(p0[8])) < 1e-3 * p0[0]  # tension along the load x axis (BC tolerance)

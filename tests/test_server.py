@@ -1,0 +1,100 @@
+"""The job server: continuous batching of RVE increments with HBM-footprint
+admission.  A result must be the same alone and in any mix, and the same as
+the single-context path's."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import PRODUCT_SO
+from solver_cpp_amd.capi import Library
+from solver_cpp_amd.geometry import polycrystal
+from solver_cpp_amd.rve import Rve, RveSpec, tension_x
+from solver_cpp_amd.serving import Server, batched_specs, rotation_to
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return Library(PRODUCT_SO, boundary=True)
+
+
+def _spec(lib, edge, grains, seed, **kw):
+    mat, ori, size = polycrystal((edge, edge, edge), grains, seed=seed)
+    return RveSpec(lib, (edge,) * 3, size, mat, ori, itmin=4, itmax=40, **kw)
+
+
+def test_footprint_and_budget(lib):
+    """(CPU) Footprints grow with the grid; a job that cannot fit is refused."""
+    small, big = _spec(lib, 8, 3, 1), _spec(lib, 16, 5, 2)
+    fs, fb = lib.sc_rve_footprint(C.byref(small.desc)), lib.sc_rve_footprint(C.byref(big.desc))
+    assert 0 < fs < fb
+    srv = Server(lib, fb // 2, 1)
+    with pytest.raises(RuntimeError, match="budget"):
+        srv.add_job(big, [tension_x(t=1.0, N=2)])
+    srv.close()
+
+
+def test_rotation_to():
+    for u in ([0, 0, 1], [1, 1, 1], [-1, 0.2, 0.1]):
+        q = np.array(rotation_to(u))
+        w, v = q[0], q[1:]
+        x = np.array([1.0, 0, 0])
+        rx = x + 2 * w * np.cross(v, x) + 2 * np.cross(v, np.cross(v, x))
+        np.testing.assert_allclose(rx, np.array(u) / np.linalg.norm(u), atol=1e-14)
+
+
+@pytest.mark.gpu
+def test_server_matches_single_context(lib):
+    """Three increments of one job on the server equal three increments of
+    the same RVE driven directly (identical code path and arithmetic)."""
+    spec = _spec(lib, 12, 6, 3)
+    load = [tension_x(t=1.5, N=3)]
+    srv = Server(lib, 1 << 36, 2)
+    job = srv.add_job(spec, load, keep_resident=True)
+    res = [srv.wait(srv.submit(job)) for _ in range(3)]
+    srv.close()
+    mat, ori, size = polycrystal((12,) * 3, 6, seed=3)
+    r = Rve(lib, (12,) * 3, size, mat, ori, itmin=4, itmax=40)
+    r.init()
+    for k, inc in enumerate(r.increments(load)):
+        assert res[k].status == 0
+        assert res[k].iterations == inc["iterations"] and res[k].evaluations == inc["evaluations"]
+        if k == 2:
+            np.testing.assert_array_equal(np.array(res[k].P_av), np.array(r.scalars().P_av))
+
+
+@pytest.mark.gpu
+def test_alone_equals_mix_and_admission(lib):
+    """Requests run alone (one worker, one at a time) and mixed (four workers,
+    budget for two resident jobs, requests interleaved) give bitwise equal
+    results; the budget is never exceeded and work does overlap."""
+    specs = batched_specs(lib, 6, seed=5, edges=(8, 12, 16), grains=(3, 12), itmin=4, itmax=40)
+    loads = [[tension_x(t=0.6, N=3, rot_q=q)] for _, q in specs]
+    alone = []
+    srv = Server(lib, 1 << 36, 1)
+    for (spec, _), load in zip(specs, loads):
+        job = srv.add_job(spec, load, keep_resident=True)
+        alone.append([srv.wait(srv.submit(job)) for _ in range(3)])
+        srv.remove_job(job)
+    srv.close()
+    fp = max(lib.sc_rve_footprint(C.byref(s.desc)) for s, _ in specs)
+    budget = 2 * fp
+    srv = Server(lib, budget, 4)
+    jobs = [srv.add_job(spec, load, keep_resident=True) for (spec, _), load in zip(specs, loads)]
+    reqs = [[] for _ in jobs]
+    for k in range(3):  # interleave the jobs' increments
+        for j, job in enumerate(jobs):
+            reqs[j].append(srv.submit(job))
+    for job in jobs:  # released as soon as their last increment is done
+        srv.remove_job(job)
+    mixed = [[srv.wait(r, 100.0) for r in rs] for rs in reqs]
+    st = srv.stats()
+    assert st.used_bytes == 0 and st.resident_jobs == 0
+    srv.close()
+    assert st.peak_bytes <= budget
+    assert st.max_concurrent >= 2
+    for a_job, m_job in zip(alone, mixed):
+        for a, m in zip(a_job, m_job):
+            assert a.status == 0 and m.status == 0
+            assert (a.iterations, a.evaluations, a.increment) == (m.iterations, m.evaluations, m.increment)
+            np.testing.assert_array_equal(np.array(a.P_av), np.array(m.P_av))
