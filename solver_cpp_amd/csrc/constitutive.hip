@@ -173,35 +173,52 @@ __device__ __forceinline__ double tau_of(const double* S, int s, double v) {
   return t;
 }
 
-// kinetics_sl (phenopowerlaw.f90:444-483): tau_pos = t, tau_neg = -t
+// kinetics_sl (phenopowerlaw.f90:444-483): tau_pos = t, tau_neg = -t, taking
+// the reciprocal slip resistances rxi = 1/xi (xi is fixed during the Newton
+// iteration, so the divisions are done once per state iteration).
+// dLp/dMp = sum_s c_s P_s (x) P_s with c_s = |gdot_s| n / |tau_s|; for a
+// compile-time exponent this is gdot0 n x^(n-1) / xi, x = |tau_s| / xi, taken
+// from the same powering chain without a division.
 template <int NE4>
-__device__ __forceinline__ void kinetics(const double* S, const double* xi, double* dg, double* c,
+__device__ __forceinline__ void kinetics(const double* S, const double* rxi, double* dg, double* c,
                                          const MatConst& mc) {
+  constexpr double TINY10 = 2.2250738585072014e-308 * 10.0;
 #pragma unroll
   for (int s = 0; s < NSLIP; ++s) {
     const double tp = tau_of(S, s, mc.v);
     const double tn = -tp;
     const double sgn = (tp > tn) ? 1.0 : -1.0;
     const double tmax = fmax(tp, tn);
-    const double g = sgn * mc.gdot0 * powe<NE4>(tmax / xi[s], mc.n_e4, mc.n);
-    dg[s] = g;
-    if (c) {
-      const double dd = (fabs(g) <= 2.2250738585072014e-308 * 10.0) ? 0.0 : g * mc.n / tmax;
-      c[s] = dd * ((g > 0.0) ? 1.0 : -1.0);  // dLp/dMp = sum c_s P_s (x) P_s
+    const double x = tmax * rxi[s];
+    if constexpr (NE4 >= 4) {
+      const double xm1 = powc<NE4 - 4>(x);
+      const double g = sgn * mc.gdot0 * (xm1 * x);
+      dg[s] = g;
+      if (c) c[s] = (fabs(g) <= TINY10) ? 0.0 : mc.gdot0 * mc.n * xm1 * rxi[s];
+    } else {
+      const double g = sgn * mc.gdot0 * powe<NE4>(x, mc.n_e4, mc.n);
+      dg[s] = g;
+      if (c) {
+        const double dd = (fabs(g) <= TINY10) ? 0.0 : g * mc.n / tmax;
+        c[s] = dd * ((g > 0.0) ? 1.0 : -1.0);
+      }
     }
   }
 }
 
-// phenopowerlaw_dotState (phenopowerlaw.f90:351-394) for state (xi, gamma)
+// phenopowerlaw_dotState (phenopowerlaw.f90:351-394) for state (xi, gamma),
+// rxi = 1/xi of the same state
 template <int NE4, int AE4>
-__device__ __forceinline__ void dot_state(const double* S, const double* st, double* dot, const MatConst& mc) {
+__device__ __forceinline__ void dot_state(const double* S, const double* st, const double* rxi, double* dot,
+                                          const MatConst& mc) {
   double dg[NSLIP];
-  kinetics<NE4>(S, st, dg, nullptr, mc);
+  kinetics<NE4>(S, rxi, dg, nullptr, mc);
+  const double rxinf = 1.0 / mc.xi_inf;
 #pragma unroll
   for (int s = 0; s < NSLIP; ++s) dg[s] = fabs(dg[s]);
 #pragma unroll
   for (int s = 0; s < NSLIP; ++s) {
-    const double x = 1.0 - st[s] / mc.xi_inf;
+    const double x = 1.0 - st[s] * rxinf;
     const double left = copysign(powe<AE4>(fabs(x), mc.a_e4, mc.a), x);
     double hg = 0.0;
 #pragma unroll
@@ -212,80 +229,86 @@ __device__ __forceinline__ void dot_state(const double* S, const double* st, dou
   for (int s = 0; s < NSLIP; ++s) dot[NSLIP + s] = dg[s];
 }
 
-// Solve J x = b (9x9, row-major) by LU with partial pivoting; row swaps are
-// done with selects so that every array index stays compile-time constant.
-__device__ __forceinline__ bool solve9_pivot(double* J, double* b) {
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
+// The Newton system J x = b of the Lp iteration lives in the traceless
+// subspace: every P_s is traceless, so J maps traceless matrices to traceless
+// ones, and b = Lp - Lp_constitutive is traceless.  Its solution is therefore
+// traceless and solves the 8x8 system on the coordinates
+//   y_k = x_k (k = 1,2,3,5,6,7, the off-diagonal row-major positions),
+//   y_0 = x_00, y_4 = x_11 (x_22 = -y_0 - y_4),
+// taking the rows 0..7 of J (row 8 = -(row 0 + row 4) on this subspace) and,
+// for the columns 0 and 4, J[:,k] - J[:,8].  The 9x9 solution is recovered
+// exactly; this is the same linear solve with an 8x8 factorisation.
+constexpr int NR = 8;
+
+// Solve J y = b (8x8, row-major) by LU with partial pivoting on memory
+// operands (dynamic indices).  Kept out of line: it runs only when the LU
+// without row exchanges refuses a pivot, and must not share registers with
+// the hot path.
+__device__ __noinline__ bool solve8_pivot(double* J, double* b) {
+  for (int k = 0; k < NR; ++k) {
     int p = k;
-    double mx = fabs(J[9 * k + k]);
-#pragma unroll
-    for (int i = k + 1; i < 9; ++i) {
-      const double v = fabs(J[9 * i + k]);
+    double mx = fabs(J[NR * k + k]);
+    for (int i = k + 1; i < NR; ++i) {
+      const double v = fabs(J[NR * i + k]);
       if (v > mx) {
         mx = v;
         p = i;
       }
     }
     if (!(mx > 0.0)) return false;
-#pragma unroll
-    for (int i = k + 1; i < 9; ++i) {
-      const bool sw = (p == i);
-#pragma unroll
-      for (int c = k; c < 9; ++c) {
-        const double a = J[9 * k + c], bb = J[9 * i + c];
-        J[9 * k + c] = sw ? bb : a;
-        J[9 * i + c] = sw ? a : bb;
+    if (p != k) {
+      for (int c = 0; c < NR; ++c) {
+        const double t = J[NR * k + c];
+        J[NR * k + c] = J[NR * p + c];
+        J[NR * p + c] = t;
       }
-      const double a = b[k], bb = b[i];
-      b[k] = sw ? bb : a;
-      b[i] = sw ? a : bb;
+      const double t = b[k];
+      b[k] = b[p];
+      b[p] = t;
     }
-    const double inv = 1.0 / J[9 * k + k];
-#pragma unroll
-    for (int i = k + 1; i < 9; ++i) {
-      const double l = J[9 * i + k] * inv;
-#pragma unroll
-      for (int c = k + 1; c < 9; ++c) J[9 * i + c] -= l * J[9 * k + c];
+    const double inv = 1.0 / J[NR * k + k];
+    for (int i = k + 1; i < NR; ++i) {
+      const double l = J[NR * i + k] * inv;
+      for (int c = k + 1; c < NR; ++c) J[NR * i + c] -= l * J[NR * k + c];
       b[i] -= l * b[k];
     }
   }
-#pragma unroll
-  for (int i = 8; i >= 0; --i) {
+  for (int i = NR - 1; i >= 0; --i) {
     double s = b[i];
-#pragma unroll
-    for (int c = i + 1; c < 9; ++c) s -= J[9 * i + c] * b[c];
-    b[i] = s / J[9 * i + i];
+    for (int c = i + 1; c < NR; ++c) s -= J[NR * i + c] * b[c];
+    b[i] = s / J[NR * i + i];
   }
   return true;
 }
 
-// LU without row exchanges; refuses (returns false, J and b untouched in
+// LU without row exchanges; refuses (returns false, J and b then hold no
 // meaning) when a pivot is smaller than 1/100 of its column's largest entry,
-// i.e. where partial pivoting would have to guard against growth.
-__device__ __forceinline__ bool solve9_nopivot(double* J, double* b) {
+// i.e. where partial pivoting would have to guard against growth.  The
+// reciprocal pivots are kept on the diagonal for the back substitution.
+__device__ __forceinline__ bool solve8_nopivot(double* J, double* b) {
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
+  for (int k = 0; k < NR; ++k) {
     double mx = 0.0;
 #pragma unroll
-    for (int i = k + 1; i < 9; ++i) mx = fmax(mx, fabs(J[9 * i + k]));
-    const double piv = J[9 * k + k];
+    for (int i = k + 1; i < NR; ++i) mx = fmax(mx, fabs(J[NR * i + k]));
+    const double piv = J[NR * k + k];
     if (!(fabs(piv) * 100.0 >= mx) || piv == 0.0) return false;
     const double inv = 1.0 / piv;
+    J[NR * k + k] = inv;
 #pragma unroll
-    for (int i = k + 1; i < 9; ++i) {
-      const double l = J[9 * i + k] * inv;
+    for (int i = k + 1; i < NR; ++i) {
+      const double l = J[NR * i + k] * inv;
 #pragma unroll
-      for (int c = k + 1; c < 9; ++c) J[9 * i + c] -= l * J[9 * k + c];
+      for (int c = k + 1; c < NR; ++c) J[NR * i + c] -= l * J[NR * k + c];
       b[i] -= l * b[k];
     }
   }
 #pragma unroll
-  for (int i = 8; i >= 0; --i) {
+  for (int i = NR - 1; i >= 0; --i) {
     double s = b[i];
 #pragma unroll
-    for (int c = i + 1; c < 9; ++c) s -= J[9 * i + c] * b[c];
-    b[i] = s / J[9 * i + i];
+    for (int c = i + 1; c < NR; ++c) s -= J[NR * i + c] * b[c];
+    b[i] = s * J[NR * i + i];
   }
   return true;
 }
@@ -302,9 +325,52 @@ __device__ __forceinline__ void store33(double* base, int64_t N, int64_t c, cons
   for (int q = 0; q < 9; ++q) base[q * N + c] = m[rm(q)];
 }
 
-// J = I + dt sum_s c_s vec(P_s) vec(G^T K_s)^T with G = Fe^T A
-__device__ __forceinline__ void assemble_jacobian(const double* Fe, const double* A, const double* c, double dt,
-                                                  double* J, const MatConst& mc) {
+// Plane addressing by a uniform plane pointer plus the lane's 32-bit byte
+// offset (scalar base + vector offset): one offset register serves every
+// plane, instead of a 64-bit lane address per plane.  Needs (c_end) * 8 < 2^32,
+// which the launcher checks.
+//
+// Every plane pointer given to ld/st/load33/store33 with a Lane must be the
+// same in all lanes (formed from kernel arguments and uniform indices only):
+// it is read from the first active lane, which keeps the compiler from
+// folding the lane offset into a 64-bit per-lane address for every plane.
+// A per-lane choice between two planes is made on the loaded values
+// (load33_sel), never on the pointer.
+struct Lane {
+  uint32_t off;  // cell index * 8
+};
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+typedef __attribute__((address_space(1))) double gdouble;  // global memory
+__device__ __forceinline__ double ld(const double* plane, Lane l) {
+  return *reinterpret_cast<const gdouble*>(uniform_u64(reinterpret_cast<uint64_t>(plane)) + l.off);
+}
+__device__ __forceinline__ void st(double* plane, Lane l, double v) {
+  *reinterpret_cast<gdouble*>(uniform_u64(reinterpret_cast<uint64_t>(plane)) + l.off) = v;
+}
+__device__ __forceinline__ void load33(const double* base, int64_t N, Lane l, double* m) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[rm(q)] = ld(base + q * N, l);
+}
+__device__ __forceinline__ void store33(double* base, int64_t N, Lane l, const double* m) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) st(base + q * N, l, m[rm(q)]);
+}
+// per-lane choice of the plane set: sel ? a : b
+__device__ __forceinline__ void load33_sel(bool sel, const double* a, const double* b, int64_t N, Lane l,
+                                           double* m) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[rm(q)] = sel ? ld(a + q * N, l) : ld(b + q * N, l);
+}
+
+// J = I + dt sum_s c_s vec(P_s) vec(G^T K_s)^T with G = Fe^T A, restricted to
+// the traceless coordinates (rows 0..7; columns 0 and 4 minus column 8), as
+// an 8x8 row-major matrix
+__device__ __forceinline__ void assemble_jacobian8(const double* Fe, const double* A, const double* c, double dt,
+                                                   double* J, const MatConst& mc) {
   double G[9];
 #pragma unroll
   for (int m = 0; m < 3; ++m)
@@ -312,7 +378,7 @@ __device__ __forceinline__ void assemble_jacobian(const double* Fe, const double
     for (int p = 0; p < 3; ++p) G[3 * m + p] = Fe[m] * A[p] + Fe[3 + m] * A[3 + p] + Fe[6 + m] * A[6 + p];
   const double kA = mc.v * mc.C44, kB = mc.v * (mc.C11 - mc.C12 - 2.0 * mc.C44);
 #pragma unroll
-  for (int k = 0; k < 81; ++k) J[k] = (k % 10 == 0) ? 1.0 : 0.0;
+  for (int k = 0; k < NR * NR; ++k) J[k] = (k % (NR + 1) == 0) ? 1.0 : 0.0;
 #pragma unroll
   for (int s = 0; s < NSLIP; ++s) {
     double W[9];  // W_s = G^T K_s
@@ -328,15 +394,17 @@ __device__ __forceinline__ void assemble_jacobian(const double* Fe, const double
         }
         W[3 * p + l] = w;
       }
+    W[0] -= W[8];
+    W[4] -= W[8];
     const double f = dt * c[s] * mc.v;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        if (psg(s, i, j) != 0) {
+        if (3 * i + j < NR && psg(s, i, j) != 0) {
           const double fs = f * (double)psg(s, i, j);
 #pragma unroll
-          for (int q = 0; q < 9; ++q) J[9 * (3 * i + j) + q] += fs * W[q];
+          for (int q = 0; q < NR; ++q) J[NR * (3 * i + j) + q] += fs * W[q];
         }
   }
 }
@@ -344,7 +412,8 @@ __device__ __forceinline__ void assemble_jacobian(const double* Fe, const double
 // Per-point output / scratch addressing
 struct PointIO {
   const MpState* s;
-  int64_t N, c;
+  int64_t N;
+  Lane l;
 };
 
 // integrateStress (phase_mechanical.f90:380-604); returns broken.  Uses the
@@ -352,40 +421,46 @@ struct PointIO {
 // successful integration straight to memory (they are read again only as
 // results or as the start of a following substep).
 template <int NE4>
-__device__ __forceinline__ bool integrate_stress(const double* F, const double* Fp0_src, double dt, const double* xi,
-                                                 double* Lp, double* S, int& newton, const PointIO& io,
-                                                 const MatConst& mc) {
-  const int64_t N = io.N, c = io.c;
+__device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, const double* Fp0_sub, double dt,
+                                                 const double* rxi, double* Lp, double* S, int& newton,
+                                                 const PointIO& io, const MatConst& mc) {
+  const int64_t N = io.N;
+  const Lane l = io.l;
   double A[9];
   {
     double Fp0[9], invFp0[9];
-    load33(Fp0_src, N, c, Fp0);
+    load33_sel(fp0_sub, Fp0_sub, io.s->Fp0, N, l, Fp0);
     if (!inv33(Fp0, invFp0, nullptr)) return true;
     mm33(F, invFp0, A);
   }
   double Lpg_old[9], dLp[9];
-  double res_old_norm = 0.0;
+  double res_old_n2 = 0.0;  // squared norms: the tests compare non-negative values
   double step = 1.0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
     Lpg_old[k] = Lp[k];
     dLp[k] = 0.0;
   }
-  double B[9];
   int it = 0;
   bool redo = false;  // repeat this iteration's solve with row pivoting
+  // B, Fe and S of an iteration are not carried across the linear solve: the
+  // converged ones are formed again from the converged Lp after the loop (S
+  // is an output only on success)
   for (;;) {
     if (!redo) {
       it++;
       if (it > mc.N_iter_Lp_max) return true;
     }
+    double Fe[9], Sl[9];
+    {
+      double B[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0 ? 1.0 : 0.0) - dt * Lp[k];
-    double Fe[9];
-    mm33(A, B, Fe);  // Fe = A B invFi_new, invFi_new = I
-    hooke_S(Fe, S, mc);
+      for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0 ? 1.0 : 0.0) - dt * Lp[k];
+      mm33(A, B, Fe);  // Fe = A B invFi_new, invFi_new = I
+    }
+    hooke_S(Fe, Sl, mc);
     double dg[NSLIP], c[NSLIP];
-    kinetics<NE4>(S, xi, dg, c, mc);
+    kinetics<NE4>(Sl, rxi, dg, c, mc);
     double res[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) res[k] = 0.0;
@@ -397,19 +472,24 @@ __device__ __forceinline__ bool integrate_stress(const double* F, const double* 
         for (int j = 0; j < 3; ++j)
           if (psg(s, i, j) != 0) res[3 * i + j] = res[3 * i + j] + dg[s] * (mc.v * (double)psg(s, i, j));
     // res holds Lp_constitutive here
-    const double nLpc = norm9(res);
+    double n2Lpc = 0.0, n2Lp = 0.0, n2res = 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) res[k] = Lp[k] - res[k];
+    for (int k = 0; k < 9; ++k) {
+      n2Lpc += res[k] * res[k];
+      n2Lp += Lp[k] * Lp[k];
+      res[k] = Lp[k] - res[k];
+      n2res += res[k] * res[k];
+    }
     if (!redo) {
-      const double nres = norm9(res);
-      const double atol = fmax(mc.eps_rel_Lp * fmax(norm9(Lp), nLpc), mc.eps_abs_Lp);
+      // |res| < max(eps_rel max(|Lp|, |Lp_c|), eps_abs), compared squared
+      const double atol = fmax(mc.eps_rel_Lp * sqrt(fmax(n2Lp, n2Lpc)), mc.eps_abs_Lp);
       bool nan = false;
 #pragma unroll
       for (int k = 0; k < 9; ++k) nan = nan || isnan(res[k]);
       if (nan) return true;
-      if (nres < atol) break;
-      if (it == 1 || nres < res_old_norm) {
-        res_old_norm = nres;
+      if (n2res < atol * atol) break;
+      if (it == 1 || n2res < res_old_n2) {
+        res_old_n2 = n2res;
 #pragma unroll
         for (int k = 0; k < 9; ++k) Lpg_old[k] = Lp[k];
         step = 1.0;
@@ -420,15 +500,27 @@ __device__ __forceinline__ bool integrate_stress(const double* F, const double* 
         continue;
       }
     }
-    double J[81];
-    assemble_jacobian(Fe, A, c, dt, J, mc);
-    const bool solved = redo ? solve9_pivot(J, res) : solve9_nopivot(J, res);
+    bool solved;
+    if (!redo) {
+      double J[NR * NR];
+      assemble_jacobian8(Fe, A, c, dt, J, mc);
+      solved = solve8_nopivot(J, res);
+    } else {  // rare: pivoted factorisation on memory operands
+      double Jm[NR * NR], bm[NR];
+      assemble_jacobian8(Fe, A, c, dt, Jm, mc);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) bm[k] = res[k];
+      solved = solve8_pivot(Jm, bm);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) res[k] = bm[k];
+    }
     if (!solved) {
       if (redo) return true;
       redo = true;
       continue;
     }
     redo = false;
+    res[8] = -(res[0] + res[4]);  // x_22 of the traceless solution
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       dLp[k] = -res[k];
@@ -436,17 +528,21 @@ __device__ __forceinline__ bool integrate_stress(const double* F, const double* 
     }
   }
   newton += it;
+  double B[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0 ? 1.0 : 0.0) - dt * Lp[k];
   double invFp_new[9], Fpn[9];
   {
     double Fp0[9], invFp0[9];
-    load33(Fp0_src, N, c, Fp0);
+    load33_sel(fp0_sub, Fp0_sub, io.s->Fp0, N, l, Fp0);
     inv33(Fp0, invFp0, nullptr);
     mm33(invFp0, B, invFp_new);
   }
   if (!inv33(invFp_new, Fpn, nullptr)) return true;
   double T1[9];
   mm33(A, B, T1);  // F invFp_new (= Fe, invFi_new = I)
-  store33(io.s->Fe, N, c, T1);
+  hooke_S(T1, S, mc);
+  store33(io.s->Fe, N, l, T1);
   {
     double iFt[9], T2[9], P[9];
 #pragma unroll
@@ -455,12 +551,12 @@ __device__ __forceinline__ bool integrate_stress(const double* F, const double* 
       for (int j = 0; j < 3; ++j) iFt[3 * i + j] = invFp_new[3 * j + i];
     mm33(S, iFt, T2);
     mm33(T1, T2, P);
-    store33(io.s->P, N, c, P);
+    store33(io.s->P, N, l, P);
   }
   const double d = pow(det33(Fpn), 1.0 / 3.0);
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fpn[k] = Fpn[k] / d;
-  store33(io.s->Fp, N, c, Fpn);
+  store33(io.s->Fp, N, l, Fpn);
   return false;
 }
 
@@ -486,16 +582,17 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
   __shared__ double h_gam[NSLIP][64];
   const int t = threadIdx.x;
   if (c >= a.c_end) return;
-  const PointIO io{&s, N, c};
+  const Lane l{(uint32_t)(c * 8)};
+  const PointIO io{&s, N, l};
   double* sub = a.sub;
   double Lp[9], S[9];
-  load33(s.Lp, N, c, Lp);
-  load33(s.S0, N, c, S);  // phase_restore: S = S0, Fp = Fp0, state = state0
+  load33(s.Lp, N, l, Lp);
+  load33(s.S0, N, l, S);  // phase_restore: S = S0, Fp = Fp0, state = state0
   double xi[NSLIP];
 #pragma unroll
   for (int k = 0; k < NSLIP; ++k) {
-    xi[k] = s.xi0[k * N + c];
-    h_gam[k][t] = s.gamma0[k * N + c];
+    xi[k] = ld(s.xi0 + (k) * N, l);
+    h_gam[k][t] = ld(s.gamma0 + (k) * N, l);
   }
   // substep start values live in memory: the increment's converged values
   // (F0, Lp0, Fp0, state0) until a partial substep succeeded, then the
@@ -503,15 +600,15 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
   bool in_sub = false;
   int newton = 0, fpi = 0;
   auto st0 = [&](int k) -> double {
-    return in_sub ? sub[(SUB_ST0 + k) * N + c] : (k < NSLIP ? s.xi0[k * N + c] : s.gamma0[(k - NSLIP) * N + c]);
+    return in_sub ? ld(sub + (SUB_ST0 + k) * N, l) : (k < NSLIP ? ld(s.xi0 + (k) * N, l) : ld(s.gamma0 + (k - NSLIP) * N, l));
   };
 
   // F of the (sub)step: F0l + step (F - F0), phase_mechanical.f90:1093-1096
   auto load_Floc = [&](double st_, double* Fl) {
     double F[9], F0l[9], F0p[9];
-    load33(s.F, N, c, F);
-    load33(in_sub ? sub + SUB_F0 * N : s.F0, N, c, F0l);
-    load33(s.F0, N, c, F0p);
+    load33(s.F, N, l, F);
+    load33_sel(in_sub, sub + SUB_F0 * N, s.F0, N, l, F0l);
+    load33(s.F0, N, l, F0p);
 #pragma unroll
     for (int k = 0; k < 9; ++k) Fl[k] = F0l[k] + st_ * (F[k] - F0p[k]);
   };
@@ -525,23 +622,23 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
       if (todo) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-          sub[(SUB_F0 + k) * N + c] = sub[(SUB_FLOC + k) * N + c];
-          sub[(SUB_LP0 + k) * N + c] = Lp[rm(k)];
-          sub[(SUB_FP0 + k) * N + c] = s.Fp[k * N + c];
+          st(sub + (SUB_F0 + k) * N, l, ld(sub + (SUB_FLOC + k) * N, l));
+          st(sub + (SUB_LP0 + k) * N, l, Lp[rm(k)]);
+          st(sub + (SUB_FP0 + k) * N, l, ld(s.Fp + (k) * N, l));
         }
 #pragma unroll
         for (int k = 0; k < NSLIP; ++k) {
-          sub[(SUB_ST0 + k) * N + c] = xi[k];
-          sub[(SUB_ST0 + NSLIP + k) * N + c] = h_gam[k][t];
+          st(sub + (SUB_ST0 + k) * N, l, xi[k]);
+          st(sub + (SUB_ST0 + NSLIP + k) * N, l, h_gam[k][t]);
         }
         in_sub = true;
       }
     } else {
       step = mc.r_cutback * step;
-      load33(s.S0, N, c, S);
+      load33(s.S0, N, l, S);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) s.Fp[k * N + c] = in_sub ? sub[(SUB_FP0 + k) * N + c] : s.Fp0[k * N + c];
-      if (step < 1.0) load33(in_sub ? sub + SUB_LP0 * N : s.Lp0, N, c, Lp);
+      for (int k = 0; k < 9; ++k) st(s.Fp + (k) * N, l, in_sub ? ld(sub + (SUB_FP0 + k) * N, l) : ld(s.Fp0 + (k) * N, l));
+      if (step < 1.0) load33_sel(in_sub, sub + SUB_LP0 * N, s.Lp0, N, l, Lp);
 #pragma unroll
       for (int k = 0; k < NSLIP; ++k) {
         xi[k] = st0(k);
@@ -553,12 +650,14 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
       if (step < 1.0) {  // keep this substep's F for the start of the next one
         double Fl[9];
         load_Floc(step, Fl);
-        store33(sub + SUB_FLOC * N, N, c, Fl);
+        store33(sub + SUB_FLOC * N, N, l, Fl);
       }
       const double sdt = step * a.dt;
       // integrateStateFPI (phase_mechanical.f90:611-693)
       bool broken = true;
-      double dot[NSTATE];
+      double dot[NSTATE], rxi[NSLIP];
+#pragma unroll
+      for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
       {
         double stt[NSTATE];
 #pragma unroll
@@ -566,7 +665,7 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
           stt[k] = xi[k];
           stt[NSLIP + k] = h_gam[k][t];
         }
-        dot_state<NE4, AE4>(S, stt, dot, mc);
+        dot_state<NE4, AE4>(S, stt, rxi, dot, mc);
       }
       bool nan = false;
 #pragma unroll
@@ -587,7 +686,9 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
           }
           double Floc[9];
           load_Floc(step, Floc);
-          broken = integrate_stress<NE4>(Floc, in_sub ? sub + SUB_FP0 * N : s.Fp0, sdt, xi, Lp, S, newton, io, mc);
+#pragma unroll
+          for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
+          broken = integrate_stress<NE4>(Floc, in_sub, sub + SUB_FP0 * N, sdt, rxi, Lp, S, newton, io, mc);
           if (broken) break;
           {
             double stt[NSTATE];
@@ -596,7 +697,7 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
               stt[k] = xi[k];
               stt[NSLIP + k] = h_gam[k][t];
             }
-            dot_state<NE4, AE4>(S, stt, dot, mc);
+            dot_state<NE4, AE4>(S, stt, rxi, dot, mc);
           }
           bool nn = false;
 #pragma unroll
@@ -637,12 +738,12 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
       converged = !broken;
     }
   }
-  store33(s.Lp, N, c, Lp);
-  store33(s.S, N, c, S);
+  store33(s.Lp, N, l, Lp);
+  store33(s.S, N, l, S);
 #pragma unroll
   for (int k = 0; k < NSLIP; ++k) {
-    s.xi[k * N + c] = xi[k];
-    s.gamma[k * N + c] = h_gam[k][t];
+    st(s.xi + (k) * N, l, xi[k]);
+    st(s.gamma + (k) * N, l, h_gam[k][t]);
   }
   if (s.iters) {
     s.iters[c] = newton;
@@ -714,8 +815,10 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) invFp0[k] = 0.0;
   }
-  double dg[NSLIP], cs[NSLIP];
-  kinetics<-1>(S, xi, dg, cs, mc);
+  double dg[NSLIP], cs[NSLIP], rxi[NSLIP];
+#pragma unroll
+  for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
+  kinetics<-1>(S, rxi, dg, cs, mc);
 
   // rhs(p,o,:,:) = Fe C(p,o,:,:) invFp^T  -> R[(p,o)][(k,l)]
   double R[81];
@@ -972,6 +1075,8 @@ void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_
   if (c_end < 0) c_end = s.N;
   if (c_begin < 0 || c_end > s.N || c_begin > c_end) throw std::runtime_error("material-point range out of bounds");
   if (c_begin == c_end) return;
+  // lanes address planes by a 32-bit byte offset (cell index * 8)
+  if (s.N > ((int64_t)1 << 29)) throw std::runtime_error("material-point planes longer than 2^29 cells");
   KernelArgs a{s, mc, dt, d_ill, d_broken, d_sub, c_begin, c_end};
   const int blocks = (int)((c_end - c_begin + 63) / 64);
   // exponents of the example material (n_sl = 20, a_sl = 2.25) are compiled

@@ -225,6 +225,7 @@ def main():
     if args.mode == "batched":
         specs = batched_specs(lib, args.rves, precision=prec, **num)
         r = run_batched(lib, specs, args.workers, budget)
+        print(f"batched: {args.workers} workers done in {r['wall']:.1f} s", file=sys.stderr, flush=True)
         seq = run_batched(lib, specs, 1, budget)
         print(json.dumps({"mode": "batched", "rves": args.rves, "workers": args.workers, "precision": args.precision,
                           "mat_point_updates_per_s": r["value"], "wall_s": r["wall"],
@@ -234,6 +235,7 @@ def main():
     else:
         reqs = serving_requests(lib, args.requests, args.rate, edge_range=(16, args.max_edge), precision=prec, **num)
         r = run_serving(lib, reqs, args.workers, budget)
+        print(f"serving: {args.workers} workers done in {r['wall']:.1f} s", file=sys.stderr, flush=True)
         seq = run_one_at_a_time(lib, reqs)
         same = max(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(r["P_av"], seq["P_av"]))
         print(json.dumps({"mode": "serving", "requests": args.requests, "rate": args.rate, "workers": args.workers,
