@@ -62,6 +62,8 @@ struct Boundary {
     allocs.clear();
     if (mp.iters) (void)hipFree(mp.iters);
     mp.iters = nullptr;
+    if (mp.repivot) (void)hipFree(mp.repivot);
+    mp.repivot = nullptr;
     if (d_ill) (void)hipFree(d_ill);
     d_ill = nullptr;
     if (st) (void)hipStreamDestroy(st);
@@ -172,6 +174,7 @@ void f_materialpoint_initDamask(void) {
     b.dPdF_dev = b.dalloc(81 * (size_t)N);
     b.sub = b.dalloc((size_t)mp_sub_planes() * N);
     SCA_HIP(hipMalloc(&mp.iters, 2 * N * sizeof(int)));
+    SCA_HIP(hipMalloc(&mp.repivot, (N + 1) * sizeof(int)));
     SCA_HIP(hipMalloc(&b.d_ill, sizeof(int)));
     // Fp0 = orientation matrix of each cell's material (material.f90:155)
     std::vector<double> om(9 * (size_t)nm), fp(9 * (size_t)N);

@@ -713,9 +713,10 @@ int sc_mp_update(int64_t n, double dt, const sc_material* m, const sc_phase_nume
     std::unique_ptr<int, void (*)(int*)> g1(d_ill, [](int* p) { (void)hipFree(p); });
     SCA_HIP(hipMalloc(&d_br, n * sizeof(int)));
     std::unique_ptr<int, void (*)(int*)> g2(d_br, [](int* p) { (void)hipFree(p); });
-    SCA_HIP(hipMalloc(&d_it, 2 * n * sizeof(int)));
+    SCA_HIP(hipMalloc(&d_it, (3 * n + 1) * sizeof(int)));
     std::unique_ptr<int, void (*)(int*)> g3(d_it, [](int* p) { (void)hipFree(p); });
     st.iters = d_it;
+    st.repivot = d_it + 2 * n;
     auto up = [&](const double* h, double* dst, int nc) {
       SCA_HIP(hipMemcpyAsync(stage.p, h, nc * n * sizeof(double), hipMemcpyHostToDevice, s.s));
       sca::launch_aos_to_soa(stage.p, dst, n, nc, s.s);

@@ -66,6 +66,7 @@ struct MpState {
   double* xi = nullptr;
   double* gamma = nullptr;
   int* iters = nullptr;      // (2,N): Newton iterations, state iterations of the last update
+  int* repivot = nullptr;    // (1 + N): list of the points the pivoting pass redoes (count first)
 };
 
 // Kernel launchers (constitutive.hip)

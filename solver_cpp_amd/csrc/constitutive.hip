@@ -54,6 +54,7 @@ struct KernelArgs {
   double dt;
   int* ill;
   int* broken;  // per point, may be null
+  int* repivot; // [0] count, [1 + i] cells for the pivoting pass
   double* sub;  // substep scratch, SUB_PLANES planes of N doubles
   int64_t c_begin, c_end;  // cells [c_begin, c_end) of the N-cell planes
 };
@@ -416,21 +417,23 @@ struct PointIO {
   Lane l;
 };
 
-// integrateStress (phase_mechanical.f90:380-604); returns broken.  Uses the
+// integrateStress (phase_mechanical.f90:380-604); returns 0 (integrated),
+// 1 (broken) or, without PIVOT, 2 when the LU without row exchanges refused a
+// pivot (the point is then redone by the PIVOT instantiation).  Uses the
 // state xi, updates S and Lp in registers and writes P, Fp, Fe of a
 // successful integration straight to memory (they are read again only as
 // results or as the start of a following substep).
-template <int NE4>
-__device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, const double* Fp0_sub, double dt,
-                                                 const double* rxi, double* Lp, double* S, int& newton,
-                                                 const PointIO& io, const MatConst& mc) {
+template <int NE4, bool PIVOT>
+__device__ __forceinline__ int integrate_stress(const double* F, bool fp0_sub, const double* Fp0_sub, double dt,
+                                                const double* rxi, double* Lp, double* S, int& newton,
+                                                const PointIO& io, const MatConst& mc) {
   const int64_t N = io.N;
   const Lane l = io.l;
   double A[9];
   {
     double Fp0[9], invFp0[9];
     load33_sel(fp0_sub, Fp0_sub, io.s->Fp0, N, l, Fp0);
-    if (!inv33(Fp0, invFp0, nullptr)) return true;
+    if (!inv33(Fp0, invFp0, nullptr)) return 1;
     mm33(F, invFp0, A);
   }
   double Lpg_old[9], dLp[9];
@@ -449,7 +452,7 @@ __device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, 
   for (;;) {
     if (!redo) {
       it++;
-      if (it > mc.N_iter_Lp_max) return true;
+      if (it > mc.N_iter_Lp_max) return 1;
     }
     double Fe[9], Sl[9];
     {
@@ -486,7 +489,7 @@ __device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, 
       bool nan = false;
 #pragma unroll
       for (int k = 0; k < 9; ++k) nan = nan || isnan(res[k]);
-      if (nan) return true;
+      if (nan) return 1;
       if (n2res < atol * atol) break;
       if (it == 1 || n2res < res_old_n2) {
         res_old_n2 = n2res;
@@ -500,12 +503,12 @@ __device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, 
         continue;
       }
     }
-    bool solved;
+    bool solved = false;
     if (!redo) {
       double J[NR * NR];
       assemble_jacobian8(Fe, A, c, dt, J, mc);
       solved = solve8_nopivot(J, res);
-    } else {  // rare: pivoted factorisation on memory operands
+    } else if constexpr (PIVOT) {  // rare: pivoted factorisation on memory operands
       double Jm[NR * NR], bm[NR];
       assemble_jacobian8(Fe, A, c, dt, Jm, mc);
 #pragma unroll
@@ -515,7 +518,8 @@ __device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, 
       for (int k = 0; k < NR; ++k) res[k] = bm[k];
     }
     if (!solved) {
-      if (redo) return true;
+      if constexpr (!PIVOT) return 2;  // the point is redone by the pivoting pass
+      if (redo) return 1;
       redo = true;
       continue;
     }
@@ -538,7 +542,7 @@ __device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, 
     inv33(Fp0, invFp0, nullptr);
     mm33(invFp0, B, invFp_new);
   }
-  if (!inv33(invFp_new, Fpn, nullptr)) return true;
+  if (!inv33(invFp_new, Fpn, nullptr)) return 1;
   double T1[9];
   mm33(A, B, T1);  // F invFp_new (= Fe, invFi_new = I)
   hooke_S(T1, S, mc);
@@ -557,7 +561,7 @@ __device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, 
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fpn[k] = Fpn[k] / d;
   store33(io.s->Fp, N, l, Fpn);
-  return false;
+  return 0;
 }
 
 }  // namespace
@@ -567,21 +571,37 @@ __device__ __forceinline__ bool integrate_stress(const double* F, bool fp0_sub, 
 // being integrated (9).  Touched only when the first full step was cut back.
 constexpr int SUB_F0 = 0, SUB_LP0 = 9, SUB_FP0 = 18, SUB_ST0 = 27, SUB_FLOC = 51, SUB_PLANES = 60;
 
-// One residual evaluation's material-point loop.
-template <int NE4, int AE4>
-__global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
+// One residual evaluation's material-point loop, in two passes.  The first
+// (mp_response_kernel) carries no pivoted factorisation, so that its
+// registers serve the common path only; a point whose Newton system the LU
+// without row exchanges refuses leaves its loops, stores no output and is
+// appended to the repivot list (its inputs are unchanged: Lp, the only plane
+// that is both read and written, is stored last).  The second
+// (mp_repivot_kernel) redoes exactly the listed points from the same inputs,
+// with the pivoted fallback, in a grid-stride loop over the list.  Points are
+// independent, so the list order does not change any result.
+//
+// repivot: [0] = number of listed points, [1 + i] = cell index.
+
+// per-lane LDS columns of one 64-lane block: FPI history
+// (dotState_last(:,1), its difference to dotState_last(:,2)) and the
+// accumulated-shear half of the state stay in LDS while the Lp Newton
+// iteration needs the registers
+struct MpLds {
+  double last1[NSTATE][64];
+  double d12[NSTATE][64];
+  double gam[NSLIP][64];
+};
+
+template <int NE4, int AE4, bool PIVOT>
+__device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, MpLds& sh) {
   const MatConst& mc = a.mc;
   const MpState& s = a.s;
   const int64_t N = s.N;
-  const int64_t c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // FPI history (dotState_last(:,1), its difference to dotState_last(:,2))
-  // and the accumulated-shear half of the state stay in LDS while the Lp
-  // Newton iteration needs the registers
-  __shared__ double h_last1[NSTATE][64];
-  __shared__ double h_d12[NSTATE][64];
-  __shared__ double h_gam[NSLIP][64];
   const int t = threadIdx.x;
-  if (c >= a.c_end) return;
+  double (&h_last1)[NSTATE][64] = sh.last1;
+  double (&h_d12)[NSTATE][64] = sh.d12;
+  double (&h_gam)[NSLIP][64] = sh.gam;
   const Lane l{(uint32_t)(c * 8)};
   const PointIO io{&s, N, l};
   double* sub = a.sub;
@@ -614,6 +634,7 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
   };
   double stepFrac = 0.0, step = 1.0 / mc.r_cutback;
   bool converged = false, todo = true;
+  bool refused = false;  // first pass only: the Newton system needs row exchanges
   while (todo) {
     if (converged) {
       stepFrac = stepFrac + step;
@@ -684,11 +705,13 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
             h_d12[k][t] = dot[k] - l2;  // last1 - last2 after the shift
             h_last1[k][t] = dot[k];
           }
-          double Floc[9];
-          load_Floc(step, Floc);
 #pragma unroll
           for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
-          broken = integrate_stress<NE4>(Floc, in_sub, sub + SUB_FP0 * N, sdt, rxi, Lp, S, newton, io, mc);
+          double Floc[9];
+          load_Floc(step, Floc);
+          const int rc = integrate_stress<NE4, PIVOT>(Floc, in_sub, sub + SUB_FP0 * N, sdt, rxi, Lp, S, newton, io, mc);
+          refused = rc == 2;
+          broken = rc != 0;
           if (broken) break;
           {
             double stt[NSTATE];
@@ -736,6 +759,14 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
         }
       }
       converged = !broken;
+      if (refused) break;  // the pivoting pass redoes this point from its inputs
+    }
+  }
+  if constexpr (!PIVOT) {
+    if (refused) {
+      const int slot = atomicAdd(a.repivot, 1);
+      a.repivot[1 + slot] = (int)c;
+      return;
     }
   }
   store33(s.Lp, N, l, Lp);
@@ -752,6 +783,25 @@ __global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
   if (a.broken) a.broken[c] = converged ? 0 : 1;
   if (!converged) atomicOr(a.ill, 1);
 }
+
+template <int NE4, int AE4>
+__global__ void __launch_bounds__(64) mp_response_kernel(KernelArgs a) {
+  __shared__ MpLds sh;
+  const int64_t c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.c_end) return;
+  mp_point<NE4, AE4, false>(a, c, sh);
+}
+
+template <int NE4, int AE4>
+__global__ void __launch_bounds__(64) mp_repivot_kernel(KernelArgs a) {
+  __shared__ MpLds sh;
+  const int64_t cnt = a.repivot[0];  // complete: the first pass ran before this launch
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+    mp_point<NE4, AE4, true>(a, a.repivot[1 + i], sh);
+}
+
+__global__ void repivot_reset_kernel(int* repivot) { repivot[0] = 0; }
+
 
 // Block partial sums of P for P_av in a fixed-order tree (deterministic):
 // block b sums cells [b*256, b*256+256) of each of the 9 planes.
@@ -1077,14 +1127,22 @@ void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_
   if (c_begin == c_end) return;
   // lanes address planes by a 32-bit byte offset (cell index * 8)
   if (s.N > ((int64_t)1 << 29)) throw std::runtime_error("material-point planes longer than 2^29 cells");
-  KernelArgs a{s, mc, dt, d_ill, d_broken, d_sub, c_begin, c_end};
+  if (!s.repivot) throw std::runtime_error("material-point state without the repivot plane");
+  KernelArgs a{s, mc, dt, d_ill, d_broken, s.repivot, d_sub, c_begin, c_end};
   const int blocks = (int)((c_end - c_begin + 63) / 64);
   // exponents of the example material (n_sl = 20, a_sl = 2.25) are compiled
   // in; any other material runs the runtime-exponent instantiation
-  if (mc.n_e4 == 80 && mc.a_e4 == 9)
+  // second pass: a fixed grid striding over the list (it holds at most the
+  // range's points)
+  const int blocks2 = std::min(blocks, 1024);
+  hipLaunchKernelGGL(repivot_reset_kernel, dim3(1), dim3(1), 0, st, s.repivot);
+  if (mc.n_e4 == 80 && mc.a_e4 == 9) {
     hipLaunchKernelGGL((mp_response_kernel<80, 9>), dim3(blocks), dim3(64), 0, st, a);
-  else
+    hipLaunchKernelGGL((mp_repivot_kernel<80, 9>), dim3(blocks2), dim3(64), 0, st, a);
+  } else {
     hipLaunchKernelGGL((mp_response_kernel<-1, -1>), dim3(blocks), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((mp_repivot_kernel<-1, -1>), dim3(blocks2), dim3(64), 0, st, a);
+  }
   SCA_HIP(hipGetLastError());
 }
 

@@ -65,6 +65,8 @@ class Ngmres {
             const std::function<int(int)>& conv, const std::function<void(int)>& set_iter);
   std::vector<double> reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
                              const std::vector<const double*>& b);
+  // final sums of the first nout partial rows, to the host
+  std::vector<double> collect(int nout, int64_t units);
   int64_t n;
   hipStream_t st;
   double *Fv = nullptr, *XM = nullptr, *FM = nullptr, *XA = nullptr, *FA = nullptr;

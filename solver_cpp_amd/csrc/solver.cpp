@@ -326,6 +326,7 @@ Rve::Rve(const sc_rve_desc& d) {
   mp.xi = dalloc(12 * N);
   mp.gamma = dalloc(12 * N);
   SCA_HIP(hipMalloc(&mp.iters, 2 * N * sizeof(int)));
+  SCA_HIP(hipMalloc(&mp.repivot, (N + 1) * sizeof(int)));
   sub = dalloc((size_t)mp_sub_planes() * N);
   X = dalloc(9 * N);
   F_last_inc = dalloc(9 * N);
@@ -417,6 +418,7 @@ Rve::~Rve() {
   fft32.release();
   for (double* p : allocs) (void)hipFree(p);
   if (mp.iters) (void)hipFree(mp.iters);
+  if (mp.repivot) (void)hipFree(mp.repivot);
   if (d_ill) (void)hipFree(d_ill);
   if (d_minmax_i) (void)hipFree(d_minmax_i);
   fft.release();
@@ -640,6 +642,16 @@ std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vecto
   return out;
 }
 
+std::vector<double> Ngmres::collect(int nout, int64_t units) {
+  std::vector<double> out(nout);
+  launch_final_reduce(partial, nout, red_blocks(), small, st);
+  if (kt) kt->end(SC_KT_REDUCE, units, st);
+  SCA_HIP(hipMemcpyAsync(out.data(), small, nout * sizeof(double), hipMemcpyDeviceToHost, st));
+  SCA_HIP(hipStreamSynchronize(st));
+  if (kt) kt->flush();
+  return out;
+}
+
 int Ngmres::solve(double* X, int msize, const std::function<void(const double*, double*)>& residual,
                   const std::function<int(int)>& conv, const std::function<void(int)>& set_iter) {
   const int max_its = 50;
@@ -668,18 +680,34 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
     launch_axpby(1.0, X, -1.0, Fv, XM, n, st);  // basic line search: XM = X - F
     if (kt) kt->end(SC_KT_REDUCE, n, st);
     residual(XM, FM);
-    // ||FM||^2, <FM, Fdot_i>, <Fdot_ivec, Fdot_i> in one pass
-    std::vector<int> ops(1 + 2 * l, 0);
-    std::vector<const double*> va(1 + 2 * l), vb(1 + 2 * l);
-    va[0] = FM;
-    vb[0] = FM;
-    for (int i = 0; i < l; ++i) {
-      va[1 + i] = FM;
-      vb[1 + i] = Fdot(i);
-      va[1 + l + i] = Fdot(ivec);
-      vb[1 + l + i] = Fdot(i);
+    // ||FM||^2, <FM, Fdot_i>, <Fdot_ivec, Fdot_i>: one pass over FM,
+    // Fdot_ivec and the history (each vector read once)
+    std::vector<double> dots(1 + 2 * l);
+    if (2 * (l + 1) <= MAX_RED_VEC) {
+      std::vector<const double*> vs(l + 1);
+      for (int i = 0; i < l; ++i) vs[i] = Fdot(i);
+      vs[l] = FM;
+      if (kt) kt->begin(SC_KT_REDUCE, st);
+      launch_shared_reduce(0, n, FM, Fdot(ivec), l + 1, vs.data(), partial, st);
+      const std::vector<double> r = collect(2 * (l + 1), n * (l + 3));
+      dots[0] = r[2 * l];
+      for (int i = 0; i < l; ++i) {
+        dots[1 + i] = r[2 * i];
+        dots[1 + l + i] = r[2 * i + 1];
+      }
+    } else {
+      std::vector<int> ops(1 + 2 * l, 0);
+      std::vector<const double*> va(1 + 2 * l), vb(1 + 2 * l);
+      va[0] = FM;
+      vb[0] = FM;
+      for (int i = 0; i < l; ++i) {
+        va[1 + i] = FM;
+        vb[1 + i] = Fdot(i);
+        va[1 + l + i] = Fdot(ivec);
+        vb[1 + l + i] = Fdot(i);
+      }
+      dots = reduce(ops, va, vb);
     }
-    const std::vector<double> dots = reduce(ops, va, vb);
     const double fMnorm = std::sqrt(dots[0]);
     if (std::isnan(fMnorm) || std::isinf(fMnorm)) return -6;  // SNES_DIVERGED_LINE_SEARCH
     {  // SNESNGMRESFormCombinedSolution_Private
@@ -709,19 +737,33 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       residual(XA, FA);
     }
     if (fminnorm > fMnorm) fminnorm = fMnorm;
-    // ||FA||^2, ||XA - XM||^2, ||Xdot_i - XA||^2 (SNESNGMRESNorms_Private)
-    std::vector<int> ops2(2 + l, 1);
-    std::vector<const double*> wa(2 + l), wb(2 + l);
-    ops2[0] = 0;
-    wa[0] = FA;
-    wb[0] = FA;
-    wa[1] = XA;
-    wb[1] = XM;
-    for (int i = 0; i < l; ++i) {
-      wa[2 + i] = Xdot(i);
-      wb[2 + i] = XA;
+    // ||FA||^2, ||XA - XM||^2, ||Xdot_i - XA||^2 (SNESNGMRESNorms_Private):
+    // FA alone, then XM and the history against XA in one pass
+    std::vector<double> nrm;
+    if (2 + l <= MAX_RED_VEC) {
+      if (kt) kt->begin(SC_KT_REDUCE, st);
+      const int op0 = 0;
+      const double* fa = FA;
+      launch_partial_reduce<double>(&op0, n, 1, &fa, &fa, partial, st);
+      std::vector<const double*> vs(l + 1);
+      vs[0] = XM;
+      for (int i = 0; i < l; ++i) vs[1 + i] = Xdot(i);
+      launch_shared_reduce(1, n, XA, nullptr, l + 1, vs.data(), partial + red_blocks(), st);
+      nrm = collect(2 + l, n * (l + 3));
+    } else {
+      std::vector<int> ops2(2 + l, 1);
+      std::vector<const double*> wa(2 + l), wb(2 + l);
+      ops2[0] = 0;
+      wa[0] = FA;
+      wb[0] = FA;
+      wa[1] = XA;
+      wb[1] = XM;
+      for (int i = 0; i < l; ++i) {
+        wa[2 + i] = Xdot(i);
+        wb[2 + i] = XA;
+      }
+      nrm = reduce(ops2, wa, wb);
     }
-    const std::vector<double> nrm = reduce(ops2, wa, wb);
     const double fAnorm = std::sqrt(nrm[0]);
     const double dnorm = std::sqrt(nrm[1]);
     double dminnorm = -1.0;

@@ -54,6 +54,11 @@ int red_blocks();
 template <typename T>
 void launch_partial_reduce(const int* ops, int64_t n, int nvec, const T* const* a, const T* const* b,
                            double* partial, hipStream_t st);
+// mode 0: out[2i] = <u, v_i>, out[2i+1] = <w, v_i>; mode 1: out[i] = sum (v_i - u)^2,
+// for i < nv, as partials (no*nv x red_blocks()) for launch_final_reduce; each
+// result bitwise equal to launch_partial_reduce's for the same pair
+void launch_shared_reduce(int mode, int64_t n, const double* u, const double* w, int nv, const double* const* v,
+                          double* partial, hipStream_t st);
 // cells of C_minMaxAvg: first cell of largest / smallest |dPdF|^2 with norms
 // within TANGENT_TIE_REL of the extreme counted as tied.  n2: N doubles of
 // scratch, pv: 2*red_blocks() doubles, pi: 2*red_blocks() long longs.

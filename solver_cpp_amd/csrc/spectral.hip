@@ -516,6 +516,67 @@ __global__ void __launch_bounds__(256) partial_reduce_kernel(RedArgs r, double* 
   if (threadIdx.x == 0) partial[(int64_t)v * RED_BLOCKS + blockIdx.x] = red[0];
 }
 
+// Reductions of several vectors against shared operands in one pass (the
+// N-GMRES inner products): MODE 0: out[2i] = <u, v_i>, out[2i+1] = <w, v_i>;
+// MODE 1: out[i] = sum (v_i - u)^2.  u and w are read once per group of
+// SHARED_GROUP vectors instead of once per product.  The per-thread element
+// order and the block tree are those of partial_reduce_kernel, so every
+// result is bitwise the one of the corresponding single reduction.
+constexpr int SHARED_GROUP = 16;
+struct SharedRedArgs {
+  int64_t n;
+  const double* u;
+  const double* w;
+  int nv;  // vectors in this launch, 1..SHARED_GROUP
+  const double* v[SHARED_GROUP];
+};
+
+template <int MODE>
+__global__ void __launch_bounds__(256) shared_reduce_kernel(SharedRedArgs r, double* partial) {
+  constexpr int NO = MODE == 0 ? 2 : 1;  // outputs per vector
+  constexpr int NACC = NO * SHARED_GROUP;
+  constexpr int CH = 8;  // accumulators per block-tree pass
+  __shared__ double red[CH][256];
+  const int t = threadIdx.x;
+  double s[NACC];
+#pragma unroll
+  for (int j = 0; j < NACC; ++j) s[j] = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < r.n; e += (int64_t)RED_BLOCKS * 256) {
+    const double x = r.u[e];
+    const double y = MODE == 0 ? r.w[e] : 0.0;
+#pragma unroll
+    for (int i = 0; i < SHARED_GROUP; ++i) {
+      if (i < r.nv) {
+        const double v = r.v[i][e];
+        if constexpr (MODE == 0) {
+          s[2 * i] += x * v;
+          s[2 * i + 1] += y * v;
+        } else {
+          const double d = v - x;
+          s[i] += d * d;
+        }
+      }
+    }
+  }
+  const int nout = NO * r.nv;
+#pragma unroll
+  for (int c0 = 0; c0 < NACC; c0 += CH) {
+    if (c0 >= nout) break;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) red[k][t] = s[c0 + k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) red[k][t] += red[k][t + w];
+      }
+      __syncthreads();
+    }
+    if (t < CH && c0 + t < nout) partial[(int64_t)(c0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
+    __syncthreads();
+  }
+}
+
 // out[v] = sum_b partial[v*nb + b] (fixed tree), one block per v
 __global__ void __launch_bounds__(256) final_reduce_kernel(const double* partial, int nb, double* out) {
   __shared__ double red[256];
@@ -810,6 +871,26 @@ void launch_partial_reduce(const int* ops, int64_t n, int nvec, const T* const* 
   hipLaunchKernelGGL(partial_reduce_kernel<T>, dim3(RED_BLOCKS, nvec), dim3(256), 0, st, r, partial);
   SCA_HIP(hipGetLastError());
 }
+void launch_shared_reduce(int mode, int64_t n, const double* u, const double* w, int nv, const double* const* v,
+                          double* partial, hipStream_t st) {
+  if (mode != 0 && mode != 1) throw std::runtime_error("shared reduction mode");
+  const int no = mode == 0 ? 2 : 1;
+  for (int g0 = 0; g0 < nv; g0 += SHARED_GROUP) {
+    SharedRedArgs r{};
+    r.n = n;
+    r.u = u;
+    r.w = mode == 0 ? w : u;
+    r.nv = std::min(SHARED_GROUP, nv - g0);
+    for (int i = 0; i < r.nv; ++i) r.v[i] = v[g0 + i];
+    double* out = partial + (int64_t)no * g0 * RED_BLOCKS;
+    if (mode == 0)
+      hipLaunchKernelGGL(shared_reduce_kernel<0>, dim3(RED_BLOCKS), dim3(256), 0, st, r, out);
+    else
+      hipLaunchKernelGGL(shared_reduce_kernel<1>, dim3(RED_BLOCKS), dim3(256), 0, st, r, out);
+    SCA_HIP(hipGetLastError());
+  }
+}
+
 template void launch_partial_reduce<double>(const int*, int64_t, int, const double* const*, const double* const*,
                                             double*, hipStream_t);
 template void launch_partial_reduce<float>(const int*, int64_t, int, const float* const*, const float* const*,
