@@ -423,19 +423,19 @@ struct PointIO {
 // state xi, updates S and Lp in registers and writes P, Fp, Fe of a
 // successful integration straight to memory (they are read again only as
 // results or as the start of a following substep).
+//
+// A = F invFp0 of the (sub)step and invFp0 (both in LDS, lane t) are formed
+// once per substep by the caller: they do not change during the state fixed
+// point.
 template <int NE4, bool PIVOT>
-__device__ __forceinline__ int integrate_stress(const double* F, bool fp0_sub, const double* Fp0_sub, double dt,
+__device__ __forceinline__ int integrate_stress(const double (*hA)[64], const double (*invFp0)[64], int t, double dt,
                                                 const double* rxi, double* Lp, double* S, int& newton,
                                                 const PointIO& io, const MatConst& mc) {
   const int64_t N = io.N;
   const Lane l = io.l;
   double A[9];
-  {
-    double Fp0[9], invFp0[9];
-    load33_sel(fp0_sub, Fp0_sub, io.s->Fp0, N, l, Fp0);
-    if (!inv33(Fp0, invFp0, nullptr)) return 1;
-    mm33(F, invFp0, A);
-  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) A[k] = hA[k][t];
   double Lpg_old[9], dLp[9];
   double res_old_n2 = 0.0;  // squared norms: the tests compare non-negative values
   double step = 1.0;
@@ -537,10 +537,10 @@ __device__ __forceinline__ int integrate_stress(const double* F, bool fp0_sub, c
   for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0 ? 1.0 : 0.0) - dt * Lp[k];
   double invFp_new[9], Fpn[9];
   {
-    double Fp0[9], invFp0[9];
-    load33_sel(fp0_sub, Fp0_sub, io.s->Fp0, N, l, Fp0);
-    inv33(Fp0, invFp0, nullptr);
-    mm33(invFp0, B, invFp_new);
+    double iF0[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) iF0[k] = invFp0[k][t];
+    mm33(iF0, B, invFp_new);
   }
   if (!inv33(invFp_new, Fpn, nullptr)) return 1;
   double T1[9];
@@ -591,6 +591,8 @@ struct MpLds {
   double last1[NSTATE][64];
   double d12[NSTATE][64];
   double gam[NSLIP][64];
+  double iFp0[9][64];  // invFp0 of the current (sub)step
+  double A[9][64];     // F_(sub)step invFp0
 };
 
 template <int NE4, int AE4, bool PIVOT>
@@ -668,10 +670,21 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
       todo = step > mc.r_cutback_min;
     }
     if (todo) {
-      if (step < 1.0) {  // keep this substep's F for the start of the next one
-        double Fl[9];
+      // A = F_(sub)step invFp0 and invFp0 are fixed during the state fixed
+      // point (integrateStress, phase_mechanical.f90:425-432)
+      bool fp0_singular;
+      {
+        double Fl[9], Fp0[9], iF0[9], A[9];
         load_Floc(step, Fl);
-        store33(sub + SUB_FLOC * N, N, l, Fl);
+        if (step < 1.0) store33(sub + SUB_FLOC * N, N, l, Fl);  // start of the next substep
+        load33_sel(in_sub, sub + SUB_FP0 * N, s.Fp0, N, l, Fp0);
+        fp0_singular = !inv33(Fp0, iF0, nullptr);
+        mm33(Fl, iF0, A);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          sh.iFp0[k][t] = iF0[k];
+          sh.A[k][t] = A[k];
+        }
       }
       const double sdt = step * a.dt;
       // integrateStateFPI (phase_mechanical.f90:611-693)
@@ -707,9 +720,8 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
           }
 #pragma unroll
           for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
-          double Floc[9];
-          load_Floc(step, Floc);
-          const int rc = integrate_stress<NE4, PIVOT>(Floc, in_sub, sub + SUB_FP0 * N, sdt, rxi, Lp, S, newton, io, mc);
+          const int rc =
+              fp0_singular ? 1 : integrate_stress<NE4, PIVOT>(sh.A, sh.iFp0, t, sdt, rxi, Lp, S, newton, io, mc);
           refused = rc == 2;
           broken = rc != 0;
           if (broken) break;
