@@ -44,11 +44,30 @@ FP64_PEAK_TFLOPS = 78.6
 #   fft:      9 real planes in, 9 half-spectrum planes out (per cell)
 #   gamma:    9 complex values in and out per wavevector
 RESPONSE_BYTES_PER_CELL = (9 * 5 + 12 * 2 + 9 * 5 + 12 * 2) * 8 + 8
-RESPONSE_FLOPS_NEWTON = 3300
+RESPONSE_FLOPS_NEWTON = 2800  # 8x8 traceless solve: Fe, Hooke, 12 slip rates, residual, assembly 1610, LU 470
 RESPONSE_FLOPS_STATE = 850
 RESPONSE_FLOPS_CELL = 300
 FFT_BYTES_PER_CELL = 9 * 8 + 9 * 16 / 2
 GAMMA_BYTES_PER_WAVEVECTOR = 2 * 9 * 16
+
+
+# HBM bytes per launch of a kernel on this workload, measured by rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes over this script and scaled by the same
+# counters on a copy kernel of known size; recorded in profiles/
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_128cubed_1000grains.md")
+
+
+def recorded_traffic(kernel: str):
+    """Bytes per launch recorded for `kernel` ('<kernel>_bytes_per_launch: N'), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                key, _, val = line.strip().strip("`").partition(":")
+                if key.strip() == f"{kernel}_bytes_per_launch":
+                    return float(val)
+    except OSError:
+        return None
+    return None
 
 
 def log(msg: str):
@@ -109,7 +128,7 @@ def roofline(res):
                     "gamma": GAMMA_BYTES_PER_WAVEVECTOR}.get(dom, 16.0)
         ach = per_unit * per_launch_units / avg_s / 1e9
         out.update(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS)
-    out["traffic"] = None
+    out["traffic"] = recorded_traffic(dom)
     return out
 
 
