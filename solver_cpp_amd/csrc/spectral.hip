@@ -499,7 +499,27 @@ __global__ void __launch_bounds__(256) partial_reduce_kernel(RedArgs r, double* 
   const T* b = reinterpret_cast<const T*>(r.b[v]);
   const int op = r.op[v];
   double s = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < r.n; t += (int64_t)RED_BLOCKS * 256) {
+  constexpr int64_t STRIDE = (int64_t)RED_BLOCKS * 256;
+  int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  // four grid-stride elements per trip with their loads issued first; the
+  // sum is accumulated in the same element order as one per trip
+  for (; t + 3 * STRIDE < r.n; t += 4 * STRIDE) {
+    double x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      x[u] = (double)a[t + u * STRIDE];
+      y[u] = (op == 2) ? 0.0 : (double)b[t + u * STRIDE];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (op == 0) s += x[u] * y[u];
+      else if (op == 1) {
+        const double d = x[u] - y[u];
+        s += d * d;
+      } else s += x[u];
+    }
+  }
+  for (; t < r.n; t += STRIDE) {
     const double x = (double)a[t];
     if (op == 0) s += x * (double)b[t];
     else if (op == 1) {
@@ -541,7 +561,36 @@ __global__ void __launch_bounds__(256) shared_reduce_kernel(SharedRedArgs r, dou
   double s[NACC];
 #pragma unroll
   for (int j = 0; j < NACC; ++j) s[j] = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < r.n; e += (int64_t)RED_BLOCKS * 256) {
+  constexpr int64_t STRIDE = (int64_t)RED_BLOCKS * 256;
+  int64_t e = (int64_t)blockIdx.x * 256 + t;
+  // two grid-stride elements per trip, loads first, same accumulation order
+  for (; e + STRIDE < r.n; e += 2 * STRIDE) {
+    double x[2], y[2], v[SHARED_GROUP][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      x[u] = r.u[e + u * STRIDE];
+      y[u] = MODE == 0 ? r.w[e + u * STRIDE] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < SHARED_GROUP; ++i)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) v[i][u] = (i < r.nv) ? r.v[i][e + u * STRIDE] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int i = 0; i < SHARED_GROUP; ++i) {
+        if (i < r.nv) {
+          if constexpr (MODE == 0) {
+            s[2 * i] += x[u] * v[i][u];
+            s[2 * i + 1] += y[u] * v[i][u];
+          } else {
+            const double d = v[i][u] - x[u];
+            s[i] += d * d;
+          }
+        }
+      }
+  }
+  for (; e < r.n; e += STRIDE) {
     const double x = r.u[e];
     const double y = MODE == 0 ? r.w[e] : 0.0;
 #pragma unroll
@@ -881,7 +930,7 @@ void launch_shared_reduce(int mode, int64_t n, const double* u, const double* w,
     r.u = u;
     r.w = mode == 0 ? w : u;
     r.nv = std::min(SHARED_GROUP, nv - g0);
-    for (int i = 0; i < r.nv; ++i) r.v[i] = v[g0 + i];
+    for (int i = 0; i < SHARED_GROUP; ++i) r.v[i] = i < r.nv ? v[g0 + i] : u;  // unused slots stay dereferenceable
     double* out = partial + (int64_t)no * g0 * RED_BLOCKS;
     if (mode == 0)
       hipLaunchKernelGGL(shared_reduce_kernel<0>, dim3(RED_BLOCKS), dim3(256), 0, st, r, out);
