@@ -417,3 +417,60 @@ def test_rotated_load_frame(lib):
     np.testing.assert_array_equal(i0, i1)
     assert np.abs(p0 - p1).max() < 1e-8 * np.abs(p1).max()
     assert p0[0] > 1e7 and max(abs(p0[4]), abs(p0[8])) < 1e-3 * p0[0]  # tension along the load x axis (BC tolerance)
+
+
+def test_update_coords_fluctuation_closed_form(lib):
+    """base_update_coords (mech_base.cpp:45-154) with a compatible field of one
+    Fourier mode: u_y(x) = A sin(k x), F = I + grad u, so F_yx = A k cos(k x)
+    at the cell centres and <F> = I.  The integrated fluctuation must give the
+    cell centres X + u(X) and the nodes X + u(X) cos(k h / 2) (the mean of u
+    over the two cells along x that share the node).  A flipped sign, a
+    transposed F or a wrong scale of the inverse transform fails this.  (The
+    reference's C++ port divides by sum |xi|^2 instead of sum conj(-xi) xi,
+    which flips u, and leaves u out of the cell centres; both follow the
+    integration F = I + grad u here.)"""
+    cells = (8, 4, 2)
+    size = (2e-5, 1e-5, 5e-6)
+    h = size[0] / cells[0]
+    k = 2 * np.pi / size[0]
+    A = 1e-7
+    R = Rve(lib, cells, size, np.zeros(int(np.prod(cells)), dtype=np.int32), np.array([[1.0, 0, 0, 0]]))
+    R.init()
+    N = int(np.prod(cells))
+    ii = np.arange(N) % cells[0]
+    xc = (ii + 0.5) * h
+    F = np.tile(np.eye(3), (N, 1, 1))
+    F[:, 1, 0] += A * k * np.cos(k * xc)
+    R.set_field("F", F.transpose(0, 2, 1).reshape(-1))  # (3,3,N) column-major per cell
+    R.update_coords()
+    xp = R.field("x_p").reshape(N, 3)
+    c = np.arange(N)
+    X = np.stack([(c % cells[0] + 0.5) * h, (c // cells[0] % cells[1] + 0.5) * size[1] / cells[1],
+                  (c // (cells[0] * cells[1]) + 0.5) * size[2] / cells[2]], axis=1)
+    expect_p = X.copy()
+    expect_p[:, 1] += A * np.sin(k * xc)
+    np.testing.assert_allclose(xp, expect_p, rtol=0, atol=1e-12 * size[0])
+    nn = np.array(cells) + 1
+    xn = R.field("x_n").reshape(-1, 3)
+    m = np.arange(xn.shape[0])
+    i = m % nn[0]
+    Xn = np.stack([i * h, (m // nn[0] % nn[1]) * size[1] / cells[1], (m // (nn[0] * nn[1])) * size[2] / cells[2]],
+                  axis=1)
+    expect_n = Xn.copy()
+    expect_n[:, 1] += A * np.sin(k * i * h) * np.cos(k * h / 2)
+    np.testing.assert_allclose(xn, expect_n, rtol=0, atol=1e-12 * size[0])
+
+
+def test_rotational_part_of_polar_product(lib):
+    """math_rotationalPart (math.f90:1117-1156): the rotation of F = R U, with
+    R a rotation and U symmetric positive definite, is R (and I for F = U)."""
+    from solver_cpp_amd.geometry import uniform_orientations
+    rng = np.random.default_rng(3)
+    for q in uniform_orientations(12, 7):
+        Rm = _om(lib, q)
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        U = Q @ np.diag(rng.uniform(0.7, 1.5, 3)) @ Q.T
+        for Fm, expect in ((Rm @ U, Rm), (U, np.eye(3))):
+            out = np.zeros(9)
+            lib.sc_rotational_part(ptr(np.ascontiguousarray(Fm.reshape(-1, order="F"))), ptr(out))
+            np.testing.assert_allclose(out.reshape(3, 3, order="F"), expect, rtol=0, atol=1e-12)
