@@ -31,6 +31,10 @@ __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
   return {a.x - b.x, a.y - b.y};
 }
 
+}  // namespace sca
+#include "fft_pencil.h"
+namespace sca {
+
 // ---------------------------------------------------------------------------
 // FFT passes
 // ---------------------------------------------------------------------------
@@ -738,6 +742,72 @@ static FftPassDesc make_desc(int n, int sign) {
   return d;
 }
 
+// Pencil kernels for the supported lengths (compile-time N); false: use the
+// generic Stockham kernels
+#define SCA_PENCIL_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(12) X(24) X(48) X(96) X(192) X(384)
+
+template <typename T, int N>
+static size_t pencil_lds() {
+  return (size_t)pencil::Shape<N>::LDS * sizeof(cpx<T>);
+}
+
+template <typename T, typename TI>
+static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, const cpx<T>* tw, hipStream_t st) {
+  switch (n) {
+#define SCA_CASE(N)                                                                                          \
+  case N: {                                                                                                  \
+    using S = pencil::Shape<N>;                                                                              \
+    const int64_t pencils = (rows + 1) / 2;                                                                  \
+    hipLaunchKernelGGL((pencil::x_r2c_kernel<T, TI, N>), dim3((unsigned)((pencils + S::P - 1) / S::P)),     \
+                       dim3(256), (pencil_lds<T, N>()), st, in, out, rows, tw);                                \
+    return true;                                                                                             \
+  }
+    SCA_PENCIL_LENGTHS(SCA_CASE)
+#undef SCA_CASE
+    default: return false;
+  }
+}
+
+template <typename T, typename TO>
+static bool pencil_x_c2r(int n, const cpx<T>* in, TO* out, int64_t rows, const cpx<T>* tw, T scale,
+                         hipStream_t st) {
+  switch (n) {
+#define SCA_CASE(N)                                                                                          \
+  case N: {                                                                                                  \
+    using S = pencil::Shape<N>;                                                                              \
+    const int64_t pencils = (rows + 1) / 2;                                                                  \
+    hipLaunchKernelGGL((pencil::x_c2r_kernel<T, TO, N>), dim3((unsigned)((pencils + S::P - 1) / S::P)),     \
+                       dim3(256), (pencil_lds<T, N>()), st, in, out, rows, tw, scale);                         \
+    return true;                                                                                             \
+  }
+    SCA_PENCIL_LENGTHS(SCA_CASE)
+#undef SCA_CASE
+    default: return false;
+  }
+}
+
+template <typename T>
+static bool pencil_strided(int n, int sign, cpx<T>* data, int ni, unsigned no, int d1, int64_t s1, int64_t s2,
+                           int64_t es, const cpx<T>* tw, hipStream_t st) {
+  switch (n) {
+#define SCA_CASE(N)                                                                                          \
+  case N: {                                                                                                  \
+    using S = pencil::Shape<N>;                                                                              \
+    const dim3 grid((unsigned)((ni + S::P - 1) / S::P), no);                                                \
+    if (sign < 0)                                                                                            \
+      hipLaunchKernelGGL((pencil::strided_kernel<T, N, -1>), grid, dim3(256), (pencil_lds<T, N>()), st, data, \
+                         ni, d1, s1, s2, es, tw);                                                            \
+    else                                                                                                     \
+      hipLaunchKernelGGL((pencil::strided_kernel<T, N, 1>), grid, dim3(256), (pencil_lds<T, N>()), st, data,  \
+                         ni, d1, s1, s2, es, tw);                                                            \
+    return true;                                                                                             \
+  }
+    SCA_PENCIL_LENGTHS(SCA_CASE)
+#undef SCA_CASE
+    default: return false;
+  }
+}
+
 template <typename T>
 void FftPlan<T>::init(const int cells_[3]) {
   for (int p = 0; p < 3; ++p) cells[p] = cells_[p];
@@ -781,7 +851,9 @@ void FftPlan<T>::forward(const TI* real, T* spec, int ncomp, hipStream_t st) con
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
   // x: rows of all components
-  {
+  if (pencil_x_r2c<T, TI>(Nx, real, S, (int64_t)ncomp * Ny * Nz, reinterpret_cast<const cpx<T>*>(tw[0]), st)) {
+    SCA_HIP(hipGetLastError());
+  } else {
     FftPassDesc d = make_desc(Nx, -1);
     const int PB = x_rows_per_block(Nx);
     const int64_t rows = (int64_t)ncomp * Ny * Nz;
@@ -797,7 +869,10 @@ template <typename T>
 void FftPlan<T>::strided_passes(void* Sv, int ncomp, int sign, hipStream_t st) const {
   cpx<T>* S = reinterpret_cast<cpx<T>*>(Sv);
   const int Ny = cells[1], Nz = cells[2];
-  if (Ny > 1) {  // y: pencils (c, z, i), stride nr
+  if (Ny > 1 && pencil_strided<T>(Ny, sign, S, nr, (unsigned)(ncomp * Nz), Nz, K, (int64_t)nr * Ny, (int64_t)nr,
+                                   reinterpret_cast<const cpx<T>*>(tw[1]), st)) {
+    SCA_HIP(hipGetLastError());
+  } else if (Ny > 1) {  // y: pencils (c, z, i), stride nr
     FftPassDesc d = make_desc(Ny, sign);
     const int PB = pencils_per_block<T>(Ny);
     dim3 grid((nr + PB - 1) / PB, (unsigned)(ncomp * Nz));
@@ -805,7 +880,10 @@ void FftPlan<T>::strided_passes(void* Sv, int ncomp, int sign, hipStream_t st) c
                        (int64_t)nr * Ny, (int64_t)nr, d, reinterpret_cast<const cpx<T>*>(tw[1]), PB);
     SCA_HIP(hipGetLastError());
   }
-  if (Nz > 1) {  // z: pencils (c, y, i), stride nr*Ny
+  if (Nz > 1 && pencil_strided<T>(Nz, sign, S, nr, (unsigned)(ncomp * Ny), Ny, K, (int64_t)nr, (int64_t)nr * Ny,
+                                   reinterpret_cast<const cpx<T>*>(tw[2]), st)) {
+    SCA_HIP(hipGetLastError());
+  } else if (Nz > 1) {  // z: pencils (c, y, i), stride nr*Ny
     FftPassDesc d = make_desc(Nz, sign);
     const int PB = pencils_per_block<T>(Nz);
     dim3 grid((nr + PB - 1) / PB, (unsigned)(ncomp * Ny));
@@ -821,6 +899,11 @@ void FftPlan<T>::backward(T* spec, TO* real, int ncomp, double scale, hipStream_
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
   strided_passes(S, ncomp, +1, st);
+  if (pencil_x_c2r<T, TO>(Nx, S, real, (int64_t)ncomp * Ny * Nz, reinterpret_cast<const cpx<T>*>(tw[0]), (T)scale,
+                          st)) {
+    SCA_HIP(hipGetLastError());
+    return;
+  }
   FftPassDesc d = make_desc(Nx, +1);
   const int PB = x_rows_per_block(Nx);
   const int64_t rows = (int64_t)ncomp * Ny * Nz;

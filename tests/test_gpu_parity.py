@@ -30,7 +30,11 @@ def test_backend(product):
 
 
 # ---- spectral stages ---------------------------------------------------------------
-FFT_CELLS = [(2, 1, 1), (4, 3, 5), (8, 8, 8), (6, 10, 7), (16, 12, 9), (32, 32, 32), (20, 14, 22)]
+# generic lengths, and the compile-time pencil lengths (2^a from 8, 3 * 2^a from
+# 12) in every axis, with odd and even numbers of x rows
+FFT_CELLS = [(2, 1, 1), (4, 3, 5), (8, 8, 8), (6, 10, 7), (16, 12, 9), (32, 32, 32), (20, 14, 22), (8, 3, 3),
+             (12, 24, 16), (128, 8, 6), (24, 48, 12), (96, 2, 64), (256, 2, 3), (512, 1, 1), (384, 3, 1),
+             (192, 2, 12)]
 
 
 @pytest.mark.parametrize("cells", FFT_CELLS)
