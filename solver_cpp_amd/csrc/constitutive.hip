@@ -550,6 +550,32 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
   double T1[9];
   mm33(A, B, T1);  // F invFp_new (= Fe, invFi_new = I)
   hooke_S(T1, S, mc);
+  return 0;
+}
+
+// Fe, P and Fp of the last successful integrate_stress (same operations on
+// the same Lp, A and invFp0, hence the same values), stored when the state
+// fixed point leaves with it; integrations that a later state iteration
+// supersedes store nothing
+__device__ __forceinline__ void store_integrated(const double (*hA)[64], const double (*invFp0)[64], int t,
+                                                 double dt, const double* Lp, const double* S, const PointIO& io) {
+  const int64_t N = io.N;
+  const Lane l = io.l;
+  double A[9], B[9], invFp_new[9], Fpn[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    A[k] = hA[k][t];
+    B[k] = (k % 4 == 0 ? 1.0 : 0.0) - dt * Lp[k];
+  }
+  {
+    double iF0[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) iF0[k] = invFp0[k][t];
+    mm33(iF0, B, invFp_new);
+  }
+  inv33(invFp_new, Fpn, nullptr);  // invertible: integrate_stress checked it
+  double T1[9];
+  mm33(A, B, T1);
   store33(io.s->Fe, N, l, T1);
   {
     double iFt[9], T2[9], P[9];
@@ -565,7 +591,6 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
 #pragma unroll
   for (int k = 0; k < 9; ++k) Fpn[k] = Fpn[k] / d;
   store33(io.s->Fp, N, l, Fpn);
-  return 0;
 }
 
 }  // namespace
@@ -729,6 +754,11 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
           refused = rc == 2;
           broken = rc != 0;
           if (broken) break;
+          // start-of-step state for the update below, loaded ahead of the
+          // slip-rate evaluation so that its latency is hidden
+          double s0[NSTATE];
+#pragma unroll
+          for (int k = 0; k < NSTATE; ++k) s0[k] = st0(k);
           {
             double stt[NSTATE];
 #pragma unroll
@@ -741,7 +771,10 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
           bool nn = false;
 #pragma unroll
           for (int k = 0; k < NSTATE; ++k) nn = nn || isnan(dot[k]);
-          if (nn) break;
+          if (nn) {  // leaves with the last integration (broken stays false)
+            store_integrated(sh.A, sh.iFp0, t, sdt, Lp, S, io);
+            break;
+          }
           double dp12 = 0.0, dp22 = 0.0, dp01 = 0.0;
 #pragma unroll
           for (int k = 0; k < NSTATE; ++k) {
@@ -761,14 +794,17 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
             const double dk = dot[k] * zeta + l1 * (1.0 - zeta);
             dot[k] = dk;
             const double cur = (k < NSLIP) ? xi[k] : h_gam[k - NSLIP][t];
-            const double r = cur - st0(k) - dk * sdt;
+            const double r = cur - s0[k] - dk * sdt;
             const double nw = cur - r;
             if (k < NSLIP) xi[k] = nw;
             else h_gam[k - NSLIP][t] = nw;
             const double at = (k < NSLIP) ? mc.atol_xi : mc.atol_gamma;
             conv = conv && (fabs(r) <= fmax(at, mc.eps_rel_state * fabs(nw)));
           }
-          if (conv) {
+          if (conv || it == mc.N_iter_state_max) {
+            // converged, or the last iteration (integrateStateFPI then
+            // returns the last integration's broken = false)
+            store_integrated(sh.A, sh.iFp0, t, sdt, Lp, S, io);
             broken = false;
             break;
           }
