@@ -658,8 +658,13 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
   auto load_Floc = [&](double st_, double* Fl) {
     double F[9], F0l[9], F0p[9];
     load33(s.F, N, l, F);
-    load33_sel(in_sub, sub + SUB_F0 * N, s.F0, N, l, F0l);
     load33(s.F0, N, l, F0p);
+    if (in_sub) {
+      load33(sub + SUB_F0 * N, N, l, F0l);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) F0l[k] = F0p[k];
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) Fl[k] = F0l[k] + st_ * (F[k] - F0p[k]);
   };
