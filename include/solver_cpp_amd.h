@@ -166,6 +166,17 @@ int sc_rve_solution(sc_rve* rve, sc_solution* out);
  * (base_update_coords, mech_base.cpp:45-154). */
 int sc_rve_update_coords(sc_rve* rve);
 
+/* Restart between increments: the state the reference writes and reads
+ * for a restart (phase_mechanical.f90:1264-1298 restartWrite/restartRead,
+ * homogenization.f90:381-438, the solver's F, F_lastInc, F_aim fields of
+ * the grid restart) -- material-point state and its converged copy, F,
+ * F_last_inc, F_dot, coordinates, and the solver scalars -- in this
+ * library's own binary file (not DADF5/HDF5).  restore needs a context
+ * created from the same descriptor; the load-case position is the caller's.
+ * Returns nonzero on I/O errors or a checkpoint of another grid. */
+int sc_rve_checkpoint(sc_rve* rve, const char* path);
+int sc_rve_restore(sc_rve* rve, const char* path);
+
 /* DamaskGrid::loop_over_loadcase, damask_grid.cpp:82-196, with stag_it_max
  * and max_cut_back from the grid numerics.  Per-increment results are
  * written to inc_iterations / inc_evaluations (length >= total increments,
@@ -261,6 +272,7 @@ typedef struct sc_server_stats {
   int64_t budget_bytes, used_bytes, peak_bytes;
   int64_t resident_jobs, completed_requests;
   int     max_concurrent;         /* most requests running at once */
+  int64_t evictions;              /* idle resident jobs checkpointed to host memory */
 } sc_server_stats;
 
 /* workers: host threads, each driving one admitted job at a time on its own

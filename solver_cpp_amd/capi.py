@@ -103,6 +103,7 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64), ("peak_bytes", C.c_int64),
         ("resident_jobs", C.c_int64), ("completed_requests", C.c_int64), ("max_concurrent", C.c_int),
+        ("evictions", C.c_int64),
     ]
 
 
@@ -161,10 +162,13 @@ _SIGNATURES = {
     "f_rotate_tensor4": (None, [_dp, _dp, _dp, _ip]),
 }
 
-# Measurement hooks of the HIP library (bound when present).
+# Entry points of the HIP library that the oracle does not have, bound when
+# present: measurement hooks and restart checkpoints.
 PROFILE_SIGNATURES = {
     "sc_rve_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "sc_rve_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
+    "sc_rve_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "sc_rve_restore": (C.c_int, [C.c_void_p, C.c_char_p]),
 }
 
 # Symbols only the product exports: the drop-in boundary of the Fortran core

@@ -5,6 +5,7 @@
 // fallback: without a HIP device every call that computes fails).
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -291,6 +292,33 @@ int sc_rve_solution(sc_rve* R, sc_solution* out) {
 int sc_rve_update_coords(sc_rve* R) {
   return timed(R, [&] {
     R->r->update_coords();
+    return 0;
+  });
+}
+
+int sc_rve_checkpoint(sc_rve* R, const char* path) {
+  return guarded([&] {
+    std::vector<char> blob;
+    R->r->save(blob);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) throw std::runtime_error(std::string("cannot write ") + path);
+    const size_t w = std::fwrite(blob.data(), 1, blob.size(), f);
+    const int c = std::fclose(f);
+    if (w != blob.size() || c != 0) throw std::runtime_error(std::string("short write to ") + path);
+    return 0;
+  });
+}
+
+int sc_rve_restore(sc_rve* R, const char* path) {
+  return guarded([&] {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw std::runtime_error(std::string("cannot read ") + path);
+    std::vector<char> blob;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) blob.insert(blob.end(), buf, buf + n);
+    std::fclose(f);
+    R->r->load(blob);
     return 0;
   });
 }

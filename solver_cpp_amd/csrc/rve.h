@@ -102,6 +102,12 @@ class Rve {
   void fetch_response_scalars(bool rotate);
   void tangent();
   void update_coords();
+  // checkpoint of the state between increments (restart, eviction): the
+  // material-point state and its converged copy, F, F_last_inc, F_dot, the
+  // coordinates and the solver scalars; load() needs a context built from
+  // the same descriptor
+  void save(std::vector<char>& blob);
+  void load(const std::vector<char>& blob);
 
   // descriptor and geometry
   sc_rve_desc desc;

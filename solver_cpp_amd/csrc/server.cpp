@@ -12,7 +12,11 @@
 // the device and new requests join as soon as a worker and memory are free.
 // Increments of one job run in submission order.  Every job's arithmetic is
 // independent of the others (no shared device state, deterministic
-// reductions), so a result is the same alone and in any mix.
+// reductions), so a result is the same alone and in any mix.  When a waiting
+// job does not fit, idle resident jobs (kept resident, no request pending)
+// are evicted, oldest first: their state is checkpointed to host memory
+// (Rve::save) and their device memory freed; their next request restores it
+// (Rve::load) instead of initialising, so eviction changes no result.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -41,6 +45,10 @@ struct Job {
   LoadCursor cur;
   bool busy = false, failed = false;
   std::deque<int64_t> pending;
+  // evicted to host memory: the checkpoint the job resumes from
+  std::vector<char> snapshot;
+  bool evicted = false;
+  uint64_t last_use = 0;  // order of the last request done (eviction picks the oldest idle job)
 };
 
 struct Request {
@@ -153,10 +161,23 @@ class Server {
     for (auto& kv : jobs_) o->resident_jobs += kv.second->rve ? 1 : 0;
     o->completed_requests = completed_;
     o->max_concurrent = max_concurrent_;
+    o->evictions = evictions_;
   }
 
  private:
   double now() const { return std::chrono::duration<double>(clk::now() - t0_).count(); }
+
+  // an idle resident job to evict when ready jobs wait for memory: oldest
+  // last use first
+  Job* pick_victim() {
+    if (ready_.empty()) return nullptr;
+    Job* v = nullptr;
+    for (auto& kv : jobs_) {
+      Job& j = *kv.second;
+      if (j.rve && !j.busy && j.pending.empty() && (!v || j.last_use < v->last_use)) v = &j;
+    }
+    return v;
+  }
 
   // first ready job (FIFO) that is resident or whose footprint fits
   bool pick(int64_t& job) {
@@ -175,23 +196,35 @@ class Server {
     for (;;) {
       int64_t jid = 0, rid = 0;
       Job* j = nullptr;
+      Job* victim = nullptr;
       bool admit = false;
       {
         std::unique_lock<std::mutex> lk(m_);
-        cv_work_.wait(lk, [&] { return stop_ || pick(jid); });
+        cv_work_.wait(lk, [&] {
+          victim = nullptr;
+          return stop_ || pick(jid) || (victim = pick_victim()) != nullptr;
+        });
         if (stop_) return;
-        j = jobs_[jid].get();
-        j->busy = true;
-        rid = j->pending.front();
-        j->pending.pop_front();
-        if (!j->rve) {
-          used_ += j->footprint;
-          peak_ = std::max(peak_, used_);
-          admit = true;
+        if (victim) {
+          victim->busy = true;  // held while it is checkpointed
+        } else {
+          j = jobs_[jid].get();
+          j->busy = true;
+          rid = j->pending.front();
+          j->pending.pop_front();
+          if (!j->rve) {
+            used_ += j->footprint;
+            peak_ = std::max(peak_, used_);
+            admit = true;
+          }
+          ++running_;
+          max_concurrent_ = std::max(max_concurrent_, running_);
+          reqs_[rid].res.t_start = now();
         }
-        ++running_;
-        max_concurrent_ = std::max(max_concurrent_, running_);
-        reqs_[rid].res.t_start = now();
+      }
+      if (victim) {
+        evict(*victim);
+        continue;
       }
       sc_request_result res{};
       res.request = rid;
@@ -199,7 +232,14 @@ class Server {
       try {
         if (admit) {
           j->rve.reset(new Rve(j->desc));
-          j->rve->init();
+          if (j->evicted) {
+            j->rve->load(j->snapshot);
+            j->snapshot.clear();
+            j->snapshot.shrink_to_fit();
+            j->evicted = false;
+          } else {
+            j->rve->init();
+          }
         }
         if (j->failed) {
           res.status = 2;
@@ -232,6 +272,7 @@ class Server {
         r.done = true;
         ++completed_;
         j->busy = false;
+        j->last_use = ++use_clock_;
         if (!j->pending.empty()) {
           ready_.push_front(jid);  // keep this job's increments flowing
         } else if (!j->keep_resident) {
@@ -246,13 +287,44 @@ class Server {
     }
   }
 
+  // checkpoint an idle resident job to host memory and free its device
+  // memory (the job was marked busy by the caller)
+  void evict(Job& v) {
+    std::vector<char> blob;
+    bool ok = true;
+    try {
+      v.rve->save(blob);
+    } catch (std::exception& e) {
+      ok = false;
+      g_err = e.what();
+    }
+    std::unique_ptr<Rve> drop;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      if (ok) {
+        v.snapshot = std::move(blob);
+        v.evicted = true;
+        drop = std::move(v.rve);
+        used_ -= v.footprint;
+        ++evictions_;
+      } else {
+        v.failed = true;
+      }
+      v.busy = false;
+      if (!v.pending.empty()) ready_.push_back(v.id);
+    }
+    drop.reset();
+    cv_work_.notify_all();
+  }
+
   std::mutex m_;
   std::condition_variable cv_work_, cv_done_;
   std::map<int64_t, std::unique_ptr<Job>> jobs_;
   std::map<int64_t, Request> reqs_;
   std::deque<int64_t> ready_;
   std::vector<std::thread> threads_;
-  int64_t budget_, used_ = 0, peak_ = 0, next_job_ = 0, next_req_ = 0, completed_ = 0;
+  int64_t budget_, used_ = 0, peak_ = 0, next_job_ = 0, next_req_ = 0, completed_ = 0, evictions_ = 0;
+  uint64_t use_clock_ = 0;
   int running_ = 0, max_concurrent_ = 0;
   bool stop_ = false;
   clk::time_point t0_;

@@ -829,6 +829,139 @@ void Rve::update_coords() {
   launch_coords_points(d_u, d_fsum, cells, size, d_xp, st);
 }
 
+// Checkpoint layout: a header (magic, version, N, cells, precision), the
+// scalar state of the solver, then the device planes in a fixed order.
+namespace {
+constexpr char CKPT_MAGIC[8] = {'S', 'C', 'A', 'M', 'D', 'C', 'K', '1'};
+struct CkptScalars {
+  double C_ref[81], C_volAvg[81], C_minMaxAvg[81], C_volAvgLastInc[81], C_minMaxAvgLastInc[81], S[81];
+  double P_av[9], P_aim[9], F_aim_dot[9], F_aim[9], F_aim_last_inc[9];
+  int stress_mask[9];
+  double rot_q[4];
+  double delta_t, last_dt, last_div_sum, err_BC, err_div;
+  int total_iter, terminally_ill, snes_iter, nfuncs, solve_evals, use32, iter_base;
+  int64_t evals32, evals64;
+  sc_stats stats;
+};
+struct CkptHeader {
+  char magic[8];
+  int64_t N;
+  int cells[3];
+  int precision;
+};
+}  // namespace
+
+void Rve::save(std::vector<char>& blob) {
+  const std::vector<std::pair<double*, int64_t>> planes = {
+      {mp.F, 9 * N},       {mp.F0, 9 * N},    {mp.Fp0, 9 * N},   {mp.Lp0, 9 * N},   {mp.S0, 9 * N},
+      {mp.xi0, 12 * N},    {mp.gamma0, 12 * N}, {mp.Lp, 9 * N},  {mp.P, 9 * N},     {mp.Fp, 9 * N},
+      {mp.S, 9 * N},       {mp.Fe, 9 * N},    {mp.xi, 12 * N},   {mp.gamma, 12 * N}, {X, 9 * N},
+      {F_last_inc, 9 * N}, {F_dot, 9 * N},    {d_u, 3 * N},      {d_xp, 3 * N},
+      {d_xn, 3 * (int64_t)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1)}};
+  int64_t doubles = 0;
+  for (auto& pl : planes) doubles += pl.second;
+  CkptHeader h{};
+  std::memcpy(h.magic, CKPT_MAGIC, 8);
+  h.N = N;
+  for (int p = 0; p < 3; ++p) h.cells[p] = cells[p];
+  h.precision = desc.precision;
+  CkptScalars sc{};
+  std::memcpy(sc.C_ref, C_ref, sizeof C_ref);
+  std::memcpy(sc.C_volAvg, C_volAvg, sizeof C_volAvg);
+  std::memcpy(sc.C_minMaxAvg, C_minMaxAvg, sizeof C_minMaxAvg);
+  std::memcpy(sc.C_volAvgLastInc, C_volAvgLastInc, sizeof C_volAvgLastInc);
+  std::memcpy(sc.C_minMaxAvgLastInc, C_minMaxAvgLastInc, sizeof C_minMaxAvgLastInc);
+  std::memcpy(sc.S, S, sizeof S);
+  std::memcpy(sc.P_av, P_av, sizeof P_av);
+  std::memcpy(sc.P_aim, P_aim, sizeof P_aim);
+  std::memcpy(sc.F_aim_dot, F_aim_dot, sizeof F_aim_dot);
+  std::memcpy(sc.F_aim, F_aim, sizeof F_aim);
+  std::memcpy(sc.F_aim_last_inc, F_aim_last_inc, sizeof F_aim_last_inc);
+  std::memcpy(sc.stress_mask, stress_mask, sizeof stress_mask);
+  std::memcpy(sc.rot_q, rot_q, sizeof rot_q);
+  sc.delta_t = delta_t;
+  sc.last_dt = last_dt;
+  sc.last_div_sum = last_div_sum;
+  sc.err_BC = err_BC;
+  sc.err_div = err_div;
+  sc.total_iter = total_iter;
+  sc.terminally_ill = terminally_ill ? 1 : 0;
+  sc.snes_iter = snes_iter;
+  sc.nfuncs = nfuncs;
+  sc.solve_evals = solve_evals;
+  sc.use32 = use32 ? 1 : 0;
+  sc.iter_base = iter_base;
+  sc.evals32 = evals32;
+  sc.evals64 = evals64;
+  sc.stats = stats;
+  blob.resize(sizeof h + sizeof sc + (size_t)doubles * sizeof(double));
+  char* w = blob.data();
+  std::memcpy(w, &h, sizeof h);
+  std::memcpy(w + sizeof h, &sc, sizeof sc);
+  size_t off = sizeof h + sizeof sc;
+  for (auto& pl : planes) {
+    SCA_HIP(hipMemcpyAsync(w + off, pl.first, pl.second * sizeof(double), hipMemcpyDeviceToHost, st));
+    off += pl.second * sizeof(double);
+  }
+  SCA_HIP(hipStreamSynchronize(st));
+}
+
+void Rve::load(const std::vector<char>& blob) {
+  CkptHeader h{};
+  CkptScalars sc{};
+  if (blob.size() < sizeof h + sizeof sc) throw std::runtime_error("checkpoint too short");
+  std::memcpy(&h, blob.data(), sizeof h);
+  if (std::memcmp(h.magic, CKPT_MAGIC, 8) != 0) throw std::runtime_error("not a solver checkpoint");
+  if (h.N != N || h.cells[0] != cells[0] || h.cells[1] != cells[1] || h.cells[2] != cells[2] ||
+      h.precision != desc.precision)
+    throw std::runtime_error("checkpoint of another grid or precision");
+  const std::vector<std::pair<double*, int64_t>> planes = {
+      {mp.F, 9 * N},       {mp.F0, 9 * N},    {mp.Fp0, 9 * N},   {mp.Lp0, 9 * N},   {mp.S0, 9 * N},
+      {mp.xi0, 12 * N},    {mp.gamma0, 12 * N}, {mp.Lp, 9 * N},  {mp.P, 9 * N},     {mp.Fp, 9 * N},
+      {mp.S, 9 * N},       {mp.Fe, 9 * N},    {mp.xi, 12 * N},   {mp.gamma, 12 * N}, {X, 9 * N},
+      {F_last_inc, 9 * N}, {F_dot, 9 * N},    {d_u, 3 * N},      {d_xp, 3 * N},
+      {d_xn, 3 * (int64_t)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1)}};
+  int64_t doubles = 0;
+  for (auto& pl : planes) doubles += pl.second;
+  if (blob.size() != sizeof h + sizeof sc + (size_t)doubles * sizeof(double))
+    throw std::runtime_error("checkpoint size does not match the grid");
+  std::memcpy(&sc, blob.data() + sizeof h, sizeof sc);
+  std::memcpy(C_ref, sc.C_ref, sizeof C_ref);
+  std::memcpy(C_volAvg, sc.C_volAvg, sizeof C_volAvg);
+  std::memcpy(C_minMaxAvg, sc.C_minMaxAvg, sizeof C_minMaxAvg);
+  std::memcpy(C_volAvgLastInc, sc.C_volAvgLastInc, sizeof C_volAvgLastInc);
+  std::memcpy(C_minMaxAvgLastInc, sc.C_minMaxAvgLastInc, sizeof C_minMaxAvgLastInc);
+  std::memcpy(S, sc.S, sizeof S);
+  std::memcpy(P_av, sc.P_av, sizeof P_av);
+  std::memcpy(P_aim, sc.P_aim, sizeof P_aim);
+  std::memcpy(F_aim_dot, sc.F_aim_dot, sizeof F_aim_dot);
+  std::memcpy(F_aim, sc.F_aim, sizeof F_aim);
+  std::memcpy(F_aim_last_inc, sc.F_aim_last_inc, sizeof F_aim_last_inc);
+  std::memcpy(stress_mask, sc.stress_mask, sizeof stress_mask);
+  std::memcpy(rot_q, sc.rot_q, sizeof rot_q);
+  delta_t = sc.delta_t;
+  last_dt = sc.last_dt;
+  last_div_sum = sc.last_div_sum;
+  err_BC = sc.err_BC;
+  err_div = sc.err_div;
+  total_iter = sc.total_iter;
+  terminally_ill = sc.terminally_ill != 0;
+  snes_iter = sc.snes_iter;
+  nfuncs = sc.nfuncs;
+  solve_evals = sc.solve_evals;
+  use32 = sc.use32 != 0;
+  iter_base = sc.iter_base;
+  evals32 = sc.evals32;
+  evals64 = sc.evals64;
+  stats = sc.stats;
+  size_t off = sizeof h + sizeof sc;
+  for (auto& pl : planes) {
+    SCA_HIP(hipMemcpyAsync(pl.first, blob.data() + off, pl.second * sizeof(double), hipMemcpyHostToDevice, st));
+    off += pl.second * sizeof(double);
+  }
+  SCA_HIP(hipStreamSynchronize(st));
+}
+
 void Rve::init() {
   std::vector<double> eye(9 * N);
   for (int q = 0; q < 9; ++q)

@@ -8,6 +8,7 @@ src/grid/damask_grid.h): ``init``, ``forward``, ``calculate_solution``,
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -152,6 +153,15 @@ class Rve:
         r = np.zeros_like(F)
         self.lib.check(self.lib.sc_rve_form_residual(self.h, ptr(F), ptr(r)), "sc_rve_form_residual")
         return r
+
+    # restart --------------------------------------------------------------------
+    def checkpoint(self, path: str):
+        """Write the state between increments to `path` (sc_rve_checkpoint)."""
+        self.lib.check(self.lib.sc_rve_checkpoint(self.h, os.fsencode(path)), "sc_rve_checkpoint")
+
+    def restore(self, path: str):
+        """Read a checkpoint of a context built from the same descriptor."""
+        self.lib.check(self.lib.sc_rve_restore(self.h, os.fsencode(path)), "sc_rve_restore")
 
     # access -------------------------------------------------------------------
     def field(self, name: str) -> np.ndarray:

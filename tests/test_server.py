@@ -98,3 +98,67 @@ def test_alone_equals_mix_and_admission(lib):
             assert a.status == 0 and m.status == 0
             assert (a.iterations, a.evaluations, a.increment) == (m.iterations, m.evaluations, m.increment)
             np.testing.assert_array_equal(np.array(a.P_av), np.array(m.P_av))
+
+
+@pytest.mark.gpu
+def test_checkpoint_restart_continues_bitwise(lib, tmp_path):
+    """A context checkpointed after three increments and restored into a new
+    context from the same descriptor continues exactly: the next increment
+    takes the same iterations and gives bitwise the same fields."""
+    mat, ori, size = polycrystal((12, 12, 12), 6, seed=3)
+    load = tension_x(t=3.0, N=6)
+    a = Rve(lib, (12, 12, 12), size, mat, ori, itmin=4, itmax=40)
+    a.init()
+    gen = a.increments([load])
+    for _ in range(3):
+        next(gen)
+    path = str(tmp_path / "rve.ckpt")
+    a.checkpoint(path)
+    b = Rve(lib, (12, 12, 12), size, mat, ori, itmin=4, itmax=40)
+    b.restore(path)
+    out = []
+    for r in (a, b):  # increment 4 of the same load case, driven identically
+        r.forward(False, True, 0.5, 0.5, 1.5, load)
+        sol = r.calculate_solution()
+        r.update_coords()
+        out.append((sol.required_iterations, sol.converged, {k: r.field(k) for k in ("F", "P", "Fp", "xi", "x_p")}))
+    (ia, ca, fa), (ib, cb, fb) = out
+    assert ia == ib and ca and cb
+    for k in fa:
+        np.testing.assert_array_equal(fa[k], fb[k], err_msg=k)
+    b.close()
+    a.close()
+
+
+@pytest.mark.gpu
+def test_eviction_to_host_changes_no_result(lib):
+    """Four kept-resident jobs advanced in rounds with HBM for two: idle jobs
+    are checkpointed to host memory and restored on their next request, and
+    every result is bitwise the one of the job run alone."""
+    specs = [_spec(lib, 12, 4 + j, 20 + j) for j in range(4)]
+    loads = [[tension_x(t=0.6, N=3)] for _ in specs]
+    alone = []
+    srv = Server(lib, 1 << 36, 1)
+    for spec, load in zip(specs, loads):
+        job = srv.add_job(spec, load, keep_resident=True)
+        alone.append([srv.wait(srv.submit(job)) for _ in range(3)])
+        srv.remove_job(job)
+    srv.close()
+    fp = max(lib.sc_rve_footprint(C.byref(s.desc)) for s in specs)
+    srv = Server(lib, 2 * fp, 2)
+    jobs = [srv.add_job(spec, load, keep_resident=True) for spec, load in zip(specs, loads)]
+    mixed = [[] for _ in jobs]
+    for k in range(3):  # one increment of every job per round; jobs are idle between rounds
+        reqs = [srv.submit(job) for job in jobs]
+        for j, r in enumerate(reqs):
+            mixed[j].append(srv.wait(r, 100.0))
+    st = srv.stats()
+    for job in jobs:
+        srv.remove_job(job)
+    srv.close()
+    assert st.evictions > 0 and st.peak_bytes <= 2 * fp
+    for a_job, m_job in zip(alone, mixed):
+        for a, m in zip(a_job, m_job):
+            assert a.status == 0 and m.status == 0
+            assert a.iterations == m.iterations and a.evaluations == m.evaluations
+            np.testing.assert_array_equal(np.array(a.P_av), np.array(m.P_av))
