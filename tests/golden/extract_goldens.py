@@ -91,6 +91,7 @@ def read(name):
 
 
 def main():
+    sys.path.insert(0, HERE)
     out = {}
     sp = read("test_spectral.cpp")
     t = in_test(sp, "SpectralTestInit")
@@ -179,6 +180,22 @@ def main():
         "expected_F_aim": tensor_colmajor(set_values(t, "expected_F_aim"), (3, 3)),
         "expected_err_BC": err_bc,
         "source": "src/grid/test/test_mech_solver_basic.cpp:232-376"}
+
+    # ---- DAMASK result file: 2x2x2 single crystal, first tension increment ---
+    # (read with the own minimal HDF5 parser hdf5_read.py; numbers only)
+    from hdf5_read import H5File
+    h5 = H5File(os.path.join(REF, "2x2x2_tensionX_material_numerics.hdf5"))
+    base = "/increment_1/phase/Aluminum/mechanical/"
+    O = h5.read(base + "O")
+    assert np.allclose(O, [1.0, 0.0, 0.0, 0.0], atol=1e-12)  # every cell: material.yaml entry 1
+    assert b"N_iter_max: 40" in h5.read("/setup/numerics.yaml")[0]
+    out["dadf5_2x2x2_increment_1"] = {
+        "cells": [2, 2, 2], "t": float(h5.attrs("/increment_1")["t/s"]),
+        "orientation": [1.0, 0.0, 0.0, 0.0], "N_iter_min": 4, "N_iter_max": 40,
+        # (N,3,3) row-major per cell as stored: [c][i][j] = X_ij
+        **{k: h5.read(base + src).tolist() for k, src in
+           (("F", "F"), ("P", "P"), ("Fp", "F_p"), ("Lp", "L_p"), ("Fe", "F_e"), ("xi", "xi_sl"))},
+        "source": "2x2x2_tensionX_material_numerics.hdf5:/increment_1 (DADF5 3.0.0-alpha8)"}
 
     with open(os.path.join(HERE, "reference_goldens.json"), "w") as f:
         json.dump(out, f, indent=None, separators=(",", ":"))

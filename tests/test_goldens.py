@@ -474,3 +474,28 @@ def test_rotational_part_of_polar_product(lib):
             out = np.zeros(9)
             lib.sc_rotational_part(ptr(np.ascontiguousarray(Fm.reshape(-1, order="F"))), ptr(out))
             np.testing.assert_allclose(out.reshape(3, 3, order="F"), expect, rtol=0, atol=1e-12)
+
+
+def test_dadf5_single_crystal_first_increment(lib, goldens):
+    """The reference's own result file 2x2x2_tensionX_material_numerics.hdf5:
+    a 2x2x2 grid of one crystal (material.yaml entry 1, identity
+    orientation) after the first increment (1 s) of tensionX (t = 10, N = 10),
+    N_iter_min/max 4/40.  The full path -- forward, N-GMRES with the stress
+    BC on P_yy and P_zz, material point -- must give its per-point F, P, Fp,
+    Lp, Fe and xi."""
+    from solver_cpp_amd.geometry import EXAMPLE_NUMERICS
+    g = goldens["dadf5_2x2x2_increment_1"]
+    assert EXAMPLE_NUMERICS == {"itmin": g["N_iter_min"], "itmax": g["N_iter_max"]}
+    R = Rve(lib, tuple(g["cells"]), (1e-5, 1e-5, 1e-5), np.zeros(8, dtype=np.int32), np.array([g["orientation"]]),
+            **EXAMPLE_NUMERICS)
+    R.init()
+    r = next(R.increments([tension_x(t=10.0, N=10)]))
+    assert r["t"] == g["t"] and r["cutbacks"] == 0
+    tol = 1e-12 if lib.backend == "cpu-oracle" else 1e-10
+    for name, ref in (("F", "F"), ("P", "P"), ("Fp", "Fp"), ("Lp", "Lp"), ("Fe", "Fe")):
+        got = R.field(name).reshape(8, 3, 3).transpose(0, 2, 1)  # (3,3,N) column-major -> [c][i][j]
+        want = np.array(g[ref])
+        assert np.abs(got - want).max() <= tol * np.abs(want).max(), name
+    xi = R.field("xi").reshape(8, 12)
+    want = np.array(g["xi"])
+    assert np.abs(xi - want).max() <= tol * np.abs(want).max()
