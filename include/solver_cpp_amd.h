@@ -177,6 +177,34 @@ int sc_rve_update_coords(sc_rve* rve);
 int sc_rve_checkpoint(sc_rve* rve, const char* path);
 int sc_rve_restore(sc_rve* rve, const char* path);
 
+/* DADF5 job result file of the context (result_init, result.f90:76-116;
+ * material_init's cell_to mappings, result.f90:492-790).  material_yaml is
+ * the text of the material configuration: it is stored under setup/, and
+ * the phase and homogenization names and the output lists
+ * (phase/<name>/mechanical/output, .../plastic/output) are read from it;
+ * NULL selects Aluminum / SX / [F, P, F_e, F_p, L_p, O] / [xi_sl]
+ * (examples/grid/material.yaml).  numerics_yaml (may be NULL) is stored
+ * under setup/.  compression = 1 shuffles and deflates datasets of at least
+ * 2^18 values, as the reference does where its HDF5 library allows it
+ * (HDF5_utilities.f90:1982-2003); every dataset carries Fletcher-32
+ * checksums either way. */
+typedef struct sc_result_desc {
+  const char* material_yaml;
+  const char* numerics_yaml;
+  int         compression;
+} sc_result_desc;
+int sc_rve_result_open(sc_rve* rve, const char* path, const sc_result_desc* desc);
+/* materialpoint_result (materialpoint.f90:221-234): increment_<inc> with
+ * t/s, phase/<phase>/mechanical/<outputs> of the last evaluation (tensors
+ * (N,3,3) with [c][i][j] = T_ij, O (N,4) quaternions of transpose(rotational
+ * part of F_e), xi_sl / gamma_sl (N,12) with slip-system labels),
+ * homogenization/<name>/mechanical, geometry/u_p (N,3) and u_n
+ * ((Nx+1)(Ny+1)Nz,3) displacements of the last update_coords.  While a file
+ * is open, sc_rve_run_loadcase writes every increment whose index in its
+ * load step is a multiple of the step's f_out (damask_grid.cpp:165-168). */
+int sc_rve_result_write(sc_rve* rve, int inc, double t);
+int sc_rve_result_close(sc_rve* rve);
+
 /* DamaskGrid::loop_over_loadcase, damask_grid.cpp:82-196, with stag_it_max
  * and max_cut_back from the grid numerics.  Per-increment results are
  * written to inc_iterations / inc_evaluations (length >= total increments,
@@ -367,6 +395,22 @@ int sc_mp_update(int64_t n, double dt, const sc_material* m, const sc_phase_nume
 void sc_qu2om(const double q[4], double om[9]);
 /* math_rotationalPart (math.f90:1117-1156), used by crystallite_orientations. */
 void sc_rotational_part(const double F[9], double R[9]);
+/* Quaternion (w >= 0) of an orientation matrix, om2qu (rotations.f90),
+ * inverse of sc_qu2om; crystallite_orientations (phase.f90:590) stores the
+ * orientation of transpose(rotational part of F_e) this way. */
+void sc_om2qu(const double om[9], double q[4]);
+
+/* HDF5 writer of the DADF5 files (the storage of HDF5_utilities.f90:
+ * 1453-1500, 1934-2037: chunked datasets with Fletcher-32, shuffle + deflate
+ * for large ones).  A host-only utility beneath sc_rve_result_*; names are
+ * "group/.../dataset" paths, parent groups are created, dims in C order. */
+typedef struct sc_dadf5 sc_dadf5;
+int sc_dadf5_create(const char* path, int compression, sc_dadf5** out);
+int sc_dadf5_dataset_f64(sc_dadf5* f, const char* name, int rank, const int64_t* dims, const double* data);
+int sc_dadf5_dataset_str(sc_dadf5* f, const char* name, const char* text);
+int sc_dadf5_attr_str(sc_dadf5* f, const char* object, const char* name, const char* value);
+int sc_dadf5_attr_f64(sc_dadf5* f, const char* object, const char* name, double value);
+int sc_dadf5_close(sc_dadf5* f);  /* writes the metadata and closes */
 
 /* ------------------------------------------------------------------------- */
 /* Drop-in boundary: the Fortran bind(C) exports the reference's C++ grid     */
@@ -385,7 +429,8 @@ void f_materialpoint_initBase(const char* material, int n1, const char* load, in
 void f_materialpoint_initDamask(void);
 /* materialpoint.f90:208-215: state0 <- state, Fp0 <- Fp, ... */
 void f_materialpoint_forward(void);
-/* materialpoint.f90:221-234 (result output; here: records the increment). */
+/* materialpoint.f90:221-234: writes increment_<inc> of the DADF5 job file
+ * created by f_materialpoint_initDamask (see sc_boundary_set_result_file). */
 void f_materialpoint_result(int* inc, double* t);
 
 /* homogenization.f90:491-510 */
@@ -419,6 +464,13 @@ void f_VTI_readDataset_int(const char* content, int length, const char* label, i
 int  sc_boundary_get_state(int field, double* out, int64_t n_values);
 int  sc_boundary_result_count(void);
 void sc_boundary_reset(void);
+/* DADF5 job file of the boundary.  Default (and after a reset): the
+ * reference's job name stem(grid)_stem(load)_stem(material)[_stem(numerics)]
+ * + ".hdf5" in the working directory (CLI.f90:311-333, result.f90:87).
+ * NULL or "" writes no file; compression as in sc_result_desc.  Takes
+ * effect at the next f_materialpoint_initDamask.  No counterpart in the
+ * reference (its job name comes from the command line). */
+int  sc_boundary_set_result_file(const char* path, int compression);
 
 /* ------------------------------------------------------------------------- */
 /* Job inputs (host)                                                          */

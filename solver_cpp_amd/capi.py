@@ -169,7 +169,23 @@ PROFILE_SIGNATURES = {
     "sc_rve_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
     "sc_rve_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sc_rve_restore": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "sc_rve_result_open": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p]),
+    "sc_rve_result_write": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "sc_rve_result_close": (C.c_int, [C.c_void_p]),
+    "sc_om2qu": (None, [_dp, _dp]),
+    "sc_dadf5_create": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "sc_dadf5_dataset_f64": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_int64), _dp]),
+    "sc_dadf5_dataset_str": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p]),
+    "sc_dadf5_attr_str": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "sc_dadf5_attr_f64": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_double]),
+    "sc_dadf5_close": (C.c_int, [C.c_void_p]),
 }
+
+
+class ResultDesc(C.Structure):
+    """sc_result_desc: DADF5 job file set-up (result.f90:76-116)."""
+
+    _fields_ = [("material_yaml", C.c_char_p), ("numerics_yaml", C.c_char_p), ("compression", C.c_int)]
 
 # Symbols only the product exports: the drop-in boundary of the Fortran core
 # (f_*), the job-input readers and the job server.
@@ -190,6 +206,7 @@ BOUNDARY_SIGNATURES = {
     "sc_boundary_get_state": (C.c_int, [C.c_int, _dp, C.c_int64]),
     "sc_boundary_result_count": (C.c_int, []),
     "sc_boundary_reset": (None, []),
+    "sc_boundary_set_result_file": (C.c_int, [C.c_char_p, C.c_int]),
     "sc_read_vti": (C.c_int, [C.c_char_p, _ip, _dp, _dp, _ip, C.c_int64]),
     "sc_read_material": (C.c_int, [C.c_char_p, C.POINTER(Material), _dp, C.c_int, _ip]),
     "sc_read_numerics": (C.c_int, [C.c_char_p, C.POINTER(GridNumerics), C.POINTER(PhaseNumerics)]),

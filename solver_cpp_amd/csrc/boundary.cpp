@@ -20,6 +20,7 @@
 
 #include "common.h"
 #include "inputs.h"
+#include "result.h"
 #include "spectral.h"
 
 namespace sca {
@@ -42,6 +43,10 @@ struct Boundary {
   std::vector<double> F0, F, P, dPdF;
   int32_t terminally_ill = 0;
   std::vector<std::pair<int, double>> results;  // (increment, time) of f_materialpoint_result
+  // DADF5 job file: the reference's job name unless set
+  bool result_default = true, result_compression = true;
+  std::string result_path;
+  std::unique_ptr<ResultFile> result;
   // device state
   hipStream_t st = nullptr;
   MatConst mc;
@@ -139,6 +144,8 @@ void f_discretization_init(int* material_at, int* n_cells, double* ip0, int* n_i
     b.node0.assign(node0, node0 + 3 * (size_t)*n_node);
     b.node = b.node0;
     b.shared_nodes_begin = shared_nodes_begin ? *shared_nodes_begin : *n_node;
+    if (b.shared_nodes_begin < 0 || b.shared_nodes_begin > *n_node)
+      throw std::runtime_error("shared_nodes_begin outside the node list");
     b.have_discretization = true;
   } catch (std::exception& e) {
     fail("f_discretization_init", e);
@@ -194,6 +201,17 @@ void f_materialpoint_initDamask(void) {
     b.P.assign(9 * (size_t)N, 0.0);
     b.dPdF.assign(81 * (size_t)N, 0.0);
     b.terminally_ill = 0;
+    // result_init and material_init's mappings (materialpoint.f90:140-150)
+    b.result.reset();
+    const std::string path = b.result_default ? solver_job_name(b.grid_path, b.load_path, b.material_path,
+                                                                b.numerics_path) + ".hdf5"
+                                              : b.result_path;
+    if (!path.empty()) {
+      ResultSetup rs = result_setup_from_material(read_file(b.material_path));
+      if (!b.numerics_path.empty()) rs.numerics_yaml = read_file(b.numerics_path);
+      rs.compression = b.result_compression;
+      b.result.reset(new ResultFile(path, rs, N));
+    }
     b.have_damask = true;
   } catch (std::exception& e) {
     fail("f_materialpoint_initDamask", e);
@@ -291,7 +309,20 @@ void f_materialpoint_forward(void) {
 
 // materialpoint.f90:221-234 writes the increment to the DADF5 job file; no
 // HDF5 output is produced here, the call records (increment, time)
-void f_materialpoint_result(int* inc, double* t) { B().results.emplace_back(*inc, *t); }
+void f_materialpoint_result(int* inc, double* t) {
+  try {
+    Boundary& b = B();
+    b.results.emplace_back(*inc, *t);
+    if (!b.result) return;
+    // discretization_result (discretization.f90:101-116)
+    std::vector<double> u_p(3 * (size_t)b.N), u_n(3 * (size_t)b.shared_nodes_begin);
+    for (size_t k = 0; k < u_p.size(); ++k) u_p[k] = b.ip[k] - b.ip0[k];
+    for (size_t k = 0; k < u_n.size(); ++k) u_n[k] = b.node[k] - b.node0[k];
+    b.result->increment(*inc, *t, b.mp, b.st, u_p, u_n);
+  } catch (std::exception& e) {
+    fail("f_materialpoint_result", e);
+  }
+}
 
 int sc_boundary_get_state(int field, double* out, int64_t n_values) {
   try {
@@ -323,5 +354,13 @@ int sc_boundary_get_state(int field, double* out, int64_t n_values) {
 int sc_boundary_result_count(void) { return boundary_result_count(); }
 
 void sc_boundary_reset(void) { g_b.reset(); }
+
+int sc_boundary_set_result_file(const char* path, int compression) {
+  Boundary& b = B();
+  b.result_default = false;
+  b.result_path = path ? path : "";
+  b.result_compression = compression != 0;
+  return 0;
+}
 
 }  // extern "C"

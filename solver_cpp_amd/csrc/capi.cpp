@@ -11,12 +11,19 @@
 #include <memory>
 #include <vector>
 
+#include "dadf5.h"
+#include "result.h"
 #include "rve.h"
 
 using sca::g_err;
 
 struct sc_rve {
   std::unique_ptr<sca::Rve> r;
+  std::unique_ptr<sca::ResultFile> result;
+};
+
+struct sc_dadf5 {
+  std::unique_ptr<sca::Dadf5Writer> w;
 };
 
 namespace {
@@ -146,43 +153,6 @@ sca::GammaHost gamma_host_table(const int cells[3], const double size[3], const 
   g.derivative = 1;  // table mode: the caller's xi1st
   g.do_gamma = do_gamma;
   return g;
-}
-
-// polar decomposition F = R U by the scaled Newton iteration
-// R <- (g R + R^-T / g) / 2, g = (|R^-1|_F / |R|_F)^(1/2) (Higham 1986)
-void rotational_part(const double* F, double* R) {
-  double X[9];
-  std::memcpy(X, F, sizeof X);
-  for (int it = 0; it < 100; ++it) {
-    double inv[9];
-    const double det = X[0] * (X[4] * X[8] - X[7] * X[5]) - X[3] * (X[1] * X[8] - X[7] * X[2]) +
-                       X[6] * (X[1] * X[5] - X[4] * X[2]);
-    if (det == 0.0) throw std::runtime_error("rotational part of a singular tensor");
-    inv[0] = (X[4] * X[8] - X[7] * X[5]) / det;
-    inv[3] = -(X[3] * X[8] - X[6] * X[5]) / det;
-    inv[6] = (X[3] * X[7] - X[6] * X[4]) / det;
-    inv[1] = -(X[1] * X[8] - X[7] * X[2]) / det;
-    inv[4] = (X[0] * X[8] - X[6] * X[2]) / det;
-    inv[7] = -(X[0] * X[7] - X[6] * X[1]) / det;
-    inv[2] = (X[1] * X[5] - X[4] * X[2]) / det;
-    inv[5] = -(X[0] * X[5] - X[3] * X[2]) / det;
-    inv[8] = (X[0] * X[4] - X[3] * X[1]) / det;
-    double nx = 0.0, ni = 0.0;
-    for (int k = 0; k < 9; ++k) {
-      nx += X[k] * X[k];
-      ni += inv[k] * inv[k];
-    }
-    const double g = std::sqrt(std::sqrt(ni / nx));
-    double Y[9], d = 0.0;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        Y[i + 3 * j] = 0.5 * (g * X[i + 3 * j] + inv[j + 3 * i] / g);
-        d = std::max(d, std::fabs(Y[i + 3 * j] - X[i + 3 * j]));
-      }
-    std::memcpy(X, Y, sizeof X);
-    if (d <= 4.0 * std::numeric_limits<double>::epsilon()) break;
-  }
-  std::memcpy(R, X, sizeof X);
 }
 
 // wall time of a solver call, accumulated in the context's statistics
@@ -322,6 +292,132 @@ int sc_rve_restore(sc_rve* R, const char* path) {
     return 0;
   });
 }
+
+// Displacements of the cell centres and of the first (Nx+1)(Ny+1)Nz nodes
+// (the reference passes that count as sharedNodesBegin for a single process,
+// discretization_grid.cpp:81), from the coordinates of the last
+// update_coords, as discretization_result writes them.
+static void rve_displacements(sca::Rve& v, std::vector<double>& u_p, std::vector<double>& u_n) {
+  const int nx = v.cells[0], ny = v.cells[1], nz = v.cells[2];
+  const int64_t N = v.N, nodes = (int64_t)(nx + 1) * (ny + 1) * nz;
+  u_p.resize(3 * (size_t)N);
+  u_n.resize(3 * (size_t)nodes);
+  SCA_HIP(hipMemcpyAsync(u_p.data(), v.d_xp, 3 * N * sizeof(double), hipMemcpyDeviceToHost, v.st));
+  SCA_HIP(hipMemcpyAsync(u_n.data(), v.d_xn, 3 * nodes * sizeof(double), hipMemcpyDeviceToHost, v.st));
+  SCA_HIP(hipStreamSynchronize(v.st));
+  const double step[3] = {v.size[0] / nx, v.size[1] / ny, v.size[2] / nz};
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int64_t c = i + (int64_t)nx * (j + (int64_t)ny * k);
+        u_p[3 * c] -= (i + 0.5) * step[0];
+        u_p[3 * c + 1] -= (j + 0.5) * step[1];
+        u_p[3 * c + 2] -= (k + 0.5) * step[2];
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const int64_t n = i + (int64_t)(nx + 1) * (j + (int64_t)(ny + 1) * k);
+        u_n[3 * n] -= i * step[0];
+        u_n[3 * n + 1] -= j * step[1];
+        u_n[3 * n + 2] -= k * step[2];
+      }
+}
+
+int sc_rve_result_open(sc_rve* R, const char* path, const sc_result_desc* d) {
+  return guarded([&] {
+    if (!path || !*path) throw std::runtime_error("no result file name");
+    sca::ResultSetup s;
+    if (d && d->material_yaml) s = sca::result_setup_from_material(d->material_yaml);
+    if (d && d->numerics_yaml) s.numerics_yaml = d->numerics_yaml;
+    s.compression = d ? d->compression != 0 : true;
+    R->r->on_result = nullptr;
+    R->result.reset();
+    R->result.reset(new sca::ResultFile(path, s, R->r->N));
+    R->r->on_result = [R](int inc, double t) {
+      std::vector<double> u_p, u_n;
+      rve_displacements(*R->r, u_p, u_n);
+      R->result->increment(inc, t, R->r->mp, R->r->st, u_p, u_n);
+    };
+    return 0;
+  });
+}
+
+int sc_rve_result_write(sc_rve* R, int inc, double t) {
+  return guarded([&] {
+    if (!R->result) throw std::runtime_error("no result file open");
+    R->r->on_result(inc, t);
+    return 0;
+  });
+}
+
+int sc_rve_result_close(sc_rve* R) {
+  return guarded([&] {
+    R->r->on_result = nullptr;
+    R->result.reset();
+    return 0;
+  });
+}
+
+int sc_dadf5_create(const char* path, int compression, sc_dadf5** out) {
+  return guarded([&] {
+    auto* h = new sc_dadf5();
+    try {
+      h->w.reset(new sca::Dadf5Writer(path, compression != 0));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return 0;
+  });
+}
+
+int sc_dadf5_dataset_f64(sc_dadf5* h, const char* name, int rank, const int64_t* dims, const double* data) {
+  return guarded([&] {
+    if (rank < 1 || rank > 8) throw std::runtime_error("dataset rank must be 1..8");
+    std::vector<uint64_t> d;
+    for (int k = 0; k < rank; ++k) {
+      if (dims[k] < 0) throw std::runtime_error("negative dataset dimension");
+      d.push_back((uint64_t)dims[k]);
+    }
+    h->w->dataset_f64(name, d, data);
+    return 0;
+  });
+}
+
+int sc_dadf5_dataset_str(sc_dadf5* h, const char* name, const char* text) {
+  return guarded([&] {
+    h->w->dataset_str(name, text);
+    return 0;
+  });
+}
+
+int sc_dadf5_attr_str(sc_dadf5* h, const char* obj, const char* name, const char* value) {
+  return guarded([&] {
+    h->w->attr_str(obj, name, value);
+    return 0;
+  });
+}
+
+int sc_dadf5_attr_f64(sc_dadf5* h, const char* obj, const char* name, double value) {
+  return guarded([&] {
+    h->w->attr_f64(obj, name, value);
+    return 0;
+  });
+}
+
+int sc_dadf5_close(sc_dadf5* h) {
+  if (!h) return 0;
+  const int rc = guarded([&] {
+    h->w->flush();
+    return 0;
+  });
+  delete h;
+  return rc;
+}
+
+void sc_om2qu(const double om[9], double q[4]) { sca::om2qu_host(om, q); }
 
 int sc_rve_run_loadcase(sc_rve* R, const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations,
                         int max_incs) {
@@ -795,7 +891,7 @@ int sc_mp_update(int64_t n, double dt, const sc_material* m, const sc_phase_nume
 void sc_qu2om(const double q[4], double om[9]) { sca::qu2om_host(q, om); }
 void sc_rotational_part(const double F[9], double R[9]) {
   try {
-    rotational_part(F, R);
+    sca::rotational_part_host(F, R);
   } catch (std::exception& e) {
     g_err = e.what();
     for (int k = 0; k < 9; ++k) R[k] = std::nan("");

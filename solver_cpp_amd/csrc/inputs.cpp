@@ -534,6 +534,18 @@ MaterialInput read_material(const std::string& text) {
   for (const char* k : {"c_1", "c_2", "c_3", "c_4", "h_0_tw-tw", "h_0_sl-tw", "h_0_tw-sl", "a_nonSchmid_110"})
     if (pl.get(k) && pl.get(k)->kind == Node::SCALAR && pl.get(k)->num() != 0.0)
       throw std::runtime_error(std::string("material: unsupported parameter ") + k);
+  // result outputs (phase_mechanical.f90:927-960, phenopowerlaw.f90:409-428)
+  auto labels = [](const Node* n, std::vector<std::string>& out) {
+    if (!n) return;
+    if (n->kind != Node::SEQ) throw std::runtime_error("material: output must be a list");
+    for (const auto& v : n->seq) out.push_back(v->value);
+  };
+  labels(mech.get("output"), mi.mechanical_output);
+  labels(pl.get("output"), mi.plastic_output);
+  const Node& homs = need(root->get("homogenization"), "homogenization");
+  if (homs.kind != Node::MAP || homs.map.size() != 1)
+    throw std::runtime_error("material: exactly one homogenization is supported");
+  mi.homogenization_name = homs.map[0].first;
   const Node& mats = need(root->get("material"), "material");
   if (mats.kind != Node::SEQ) throw std::runtime_error("material: 'material' must be a list");
   for (const auto& entry : mats.seq) {

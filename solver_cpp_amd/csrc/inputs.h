@@ -26,6 +26,9 @@ std::string read_file(const std::string& path);
 struct MaterialInput {
   sc_material material;
   std::string phase_name;
+  std::string homogenization_name;
+  std::vector<std::string> mechanical_output;  // phase/<name>/mechanical/output
+  std::vector<std::string> plastic_output;     // phase/<name>/mechanical/plastic/output
   std::vector<double> orientations;  // (4, n_materials), w x y z
 };
 MaterialInput read_material(const std::string& text);

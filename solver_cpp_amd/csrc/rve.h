@@ -108,6 +108,10 @@ class Rve {
   // the same descriptor
   void save(std::vector<char>& blob);
   void load(const std::vector<char>& blob);
+  // called after a converged increment whose index in its load step is a
+  // multiple of the step's f_out, with the total increment count and time
+  // (damask_grid.cpp:165-168)
+  std::function<void(int, double)> on_result;
 
   // descriptor and geometry
   sc_rve_desc desc;

@@ -1113,6 +1113,7 @@ int Rve::next_increment(const sc_loadstep* steps, int n_steps, LoadCursor& cur, 
   }
   cur.cut_back_level = std::max(0, (int)cur.cut_back - 1);  // damask_grid.cpp:153
   stats.increments++;
+  if (on_result && stp.f_out > 0 && inc % stp.f_out == 0) on_result(cur.total_inc, cur.t);
   if (iterations) *iterations = iters;
   if (evaluations) *evaluations = evals;
   cur.inc++;

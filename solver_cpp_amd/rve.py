@@ -163,6 +163,25 @@ class Rve:
         """Read a checkpoint of a context built from the same descriptor."""
         self.lib.check(self.lib.sc_rve_restore(self.h, os.fsencode(path)), "sc_rve_restore")
 
+    # DADF5 results -----------------------------------------------------------------
+    def result_open(self, path: str, material_yaml: str = None, numerics_yaml: str = None,
+                    compression: bool = True):
+        """Create the DADF5 job file (sc_rve_result_open); run_loadcase then
+        writes increments at each load step's f_out cadence."""
+        from .capi import ResultDesc
+
+        d = ResultDesc(material_yaml.encode() if material_yaml is not None else None,
+                       numerics_yaml.encode() if numerics_yaml is not None else None, int(compression))
+        self._result_desc = d
+        self.lib.check(self.lib.sc_rve_result_open(self.h, os.fsencode(path), C.byref(d)), "sc_rve_result_open")
+
+    def result_write(self, inc: int, t: float):
+        """Write increment_<inc> of the open job file (materialpoint_result)."""
+        self.lib.check(self.lib.sc_rve_result_write(self.h, int(inc), float(t)), "sc_rve_result_write")
+
+    def result_close(self):
+        self.lib.check(self.lib.sc_rve_result_close(self.h), "sc_rve_result_close")
+
     # access -------------------------------------------------------------------
     def field(self, name: str) -> np.ndarray:
         n = {"xi": 12, "gamma": 12, "x_p": 3, "iterations": 2}.get(name, 9) * self.N

@@ -195,7 +195,41 @@ def main():
         # (N,3,3) row-major per cell as stored: [c][i][j] = X_ij
         **{k: h5.read(base + src).tolist() for k, src in
            (("F", "F"), ("P", "P"), ("Fp", "F_p"), ("Lp", "L_p"), ("Fe", "F_e"), ("xi", "xi_sl"))},
+        "O": O.tolist(),
+        "u_n": h5.read("/increment_1/geometry/u_n").tolist(),
         "source": "2x2x2_tensionX_material_numerics.hdf5:/increment_1 (DADF5 3.0.0-alpha8)"}
+
+    # ---- the same file's storage: every object, its shape and attribute
+    # names, and the stored Fletcher-32 checksum of each single-chunk dataset
+    # (values are in the group above; the setup texts are kept here)
+    import struct
+    objects, checksums = [], {}
+
+    def walk(path):
+        oh = h5.resolve(path)
+        kids = h5.children(oh)
+        attrs = sorted(h5.attrs(path)) if path != "/" else sorted(h5.attrs("/"))
+        msgs = h5.messages(oh)
+        layout = [d for t, d in msgs if t == 0x08]
+        if layout:
+            dims = h5._dataspace([d for t, d in msgs if t == 0x01][0])
+            objects.append({"path": path, "shape": list(dims), "attrs": attrs,
+                            "layout": ["compact", "contiguous", "chunked"][layout[0][1]]})
+            if layout[0][1] == 2:
+                chunks = list(h5._chunks(struct.unpack_from("<Q", layout[0], 3)[0], layout[0][2] - 1))
+                if len(chunks) == 1:
+                    checksums[path] = struct.unpack_from("<I", chunks[0][1], len(chunks[0][1]) - 4)[0]
+        else:
+            objects.append({"path": path, "attrs": attrs, "group": True})
+            for k in sorted(kids):
+                walk(path.rstrip("/") + "/" + k)
+
+    walk("/")
+    out["dadf5_2x2x2_storage"] = {
+        "objects": objects, "fletcher32": checksums,
+        "material_yaml": h5.read("/setup/material.yaml")[0].decode(),
+        "numerics_yaml": h5.read("/setup/numerics.yaml")[0].decode(),
+        "source": "2x2x2_tensionX_material_numerics.hdf5 (DADF5 3.0.0-alpha8)"}
 
     with open(os.path.join(HERE, "reference_goldens.json"), "w") as f:
         json.dump(out, f, indent=None, separators=(",", ":"))

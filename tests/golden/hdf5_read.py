@@ -21,6 +21,24 @@ SIG = b"\x89HDF\r\n\x1a\n"
 UNDEF = 0xFFFFFFFFFFFFFFFF
 
 
+def fletcher32(b: bytes) -> int:
+    """HDF5's Fletcher-32: 16-bit big-endian words (an odd last byte as the
+    high byte of a word), both sums folded to 16 bits every 360 words."""
+    a = np.frombuffer(b, dtype=np.uint8)
+    n = a.size
+    words = (a[: n - n % 2].astype(np.uint64).reshape(-1, 2) * np.array([256, 1], np.uint64)).sum(1)
+    if n % 2:
+        words = np.append(words, np.uint64(int(a[-1]) << 8))
+    fold = lambda s: (s & 0xFFFF) + (s >> 16)
+    s1 = s2 = 0
+    for k in range(0, words.size, 360):
+        blk = words[k:k + 360]
+        c = np.cumsum(blk, dtype=np.uint64) + np.uint64(s1)
+        s1 = fold(int(c[-1]))
+        s2 = fold(s2 + int(c.sum(dtype=np.uint64)))
+    return (fold(s2) << 16) | fold(s1)
+
+
 class H5File:
     def __init__(self, path: str):
         with open(path, "rb") as f:
@@ -244,7 +262,9 @@ class H5File:
     @staticmethod
     def _unfilter(data: bytes, filters, itemsize: int) -> bytes:
         for fid, cv in reversed(filters):
-            if fid == 3:  # Fletcher-32: checksum in the last 4 bytes
+            if fid == 3:  # Fletcher-32: checksum in the last 4 bytes, little endian
+                if struct.unpack_from("<I", data, len(data) - 4)[0] != fletcher32(data[:-4]):
+                    raise ValueError("Fletcher-32 checksum mismatch")
                 data = data[:-4]
             elif fid == 1:  # deflate
                 data = zlib.decompress(data)

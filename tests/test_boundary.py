@@ -43,9 +43,11 @@ def _oracle_update(oracle, n, dt, F, F0, Fp0, Lp0, S0, xi0, g0, Lp):
     return out
 
 
-def test_boundary_increments_match_oracle(oracle):
+def test_boundary_increments_match_oracle(oracle, tmp_path):
     lib = Library(PRODUCT_SO, boundary=True)
     lib.sc_boundary_reset()
+    job = str(tmp_path / "grid_tensionX_material_numerics.hdf5")
+    lib.sc_boundary_set_result_file(job.encode(), 1)
     cells, size, mat0 = _geometry(lib)
     N = int(np.prod(cells))
     mpath = os.path.join(GOLDEN, "material.yaml").encode()
@@ -105,6 +107,18 @@ def test_boundary_increments_match_oracle(oracle):
         st = dict(F0=np.ascontiguousarray(Fcur), Fp0=ref["Fp"], Lp0=ref["Lp"], S0=ref["S"], xi0=ref["xi"],
                   g0=ref["gamma"], Lp=ref["Lp"])
     assert lib.sc_boundary_result_count() == 3
+    # the DADF5 job file (materialpoint_result): one group per call, P of the
+    # last call as the host saw it, no displacement without coordinate updates
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from hdf5_read import H5File
+    h5 = H5File(job)
+    assert h5.keys("/") == ["cell_to", "increment_1", "increment_2", "increment_3", "setup"]
+    assert h5.attrs("/increment_2")["t/s"] == 2 * dt
+    np.testing.assert_array_equal(h5.read("/increment_3/phase/Aluminum/mechanical/P"),
+                                  P.reshape(N, 3, 3).transpose(0, 2, 1))
+    assert not h5.read("/increment_3/geometry/u_n").any() and h5.read("/increment_3/geometry/u_p").shape == (N, 3)
+    assert h5.read("/setup/numerics.yaml")[0].decode() == open(npath.decode()).read()
     # a sub-range touches only its cells
     P_before = P.copy()
     F[:] = Fcur * 1.0001
