@@ -335,6 +335,7 @@ int sc_rve_result_open(sc_rve* R, const char* path, const sc_result_desc* d) {
     R->result.reset();
     R->result.reset(new sca::ResultFile(path, s, R->r->N));
     R->r->on_result = [R](int inc, double t) {
+      R->r->sync_response_F();
       std::vector<double> u_p, u_n;
       rve_displacements(*R->r, u_p, u_n);
       R->result->increment(inc, t, R->r->mp, R->r->st, u_p, u_n);
@@ -432,6 +433,7 @@ int sc_rve_form_residual(sc_rve* R, const double* F, double* r) {
     a.put(F, v.st);
     sca::launch_aos_to_soa(a.p, soa.p, N, 9, v.st);
     v.form_residual(soa.p, res.p);
+    v.sync_response_F();
     sca::launch_soa_to_aos(res.p, a.p, N, 9, v.st);
     a.get(r, v.st);
     SCA_HIP(hipStreamSynchronize(v.st));

@@ -22,6 +22,7 @@ void sym_lstsq(int l, std::vector<double> H, std::vector<double>& b);
 // reduction results
 enum {
   SM_SUMS = 100,  // 9: P sums of the last response
+  SM_ILL = 109,   // int in a double slot: terminally-ill flag of the last response
   SM_DIV = 110,   // 1: divergence sum
   SM_AIM = 120,   // 9: DC of the Gamma convolution
   SM_AV = 130,    // 9: average rate
@@ -69,13 +70,21 @@ class Ngmres {
   std::vector<double> collect(int nout, int64_t units);
   int64_t n;
   hipStream_t st;
-  double *Fv = nullptr, *XM = nullptr, *FM = nullptr, *XA = nullptr, *FA = nullptr;
   double *partial = nullptr, *small = nullptr;
   KTimer* kt = nullptr;
 
  private:
-  std::vector<double*> bufs, hist;
-  double* slot(int k);
+  // The vectors of a solve are buffers of one pool, referenced by the
+  // iterate (Xc, its residual Fv), the trial points and residuals of an
+  // iteration and the history slots.  Accepting a point or recording it in
+  // the history moves a reference instead of copying the vector; a buffer
+  // is written only while nothing else references it.  The caller's X is
+  // referenced until the first accepted point and receives the solution at
+  // the end (one copy per solve).
+  double *Fv = nullptr, *Xc = nullptr, *XM = nullptr, *FM = nullptr, *XA = nullptr, *FA = nullptr;
+  std::vector<double*> Fh, Xh;
+  std::vector<double*> bufs, pool;
+  double* take();  // a pool buffer that nothing references
 };
 
 // position in a load case (the loop variables of loop_over_loadcase)
@@ -102,6 +111,9 @@ class Rve {
   void fetch_response_scalars(bool rotate);
   void tangent();
   void update_coords();
+  // mp.F <- the F of the last constitutive response (the response reads F
+  // from the solver's vector; the copy is made once, when mp.F is used)
+  void sync_response_F();
   // checkpoint of the state between increments (restart, eviction): the
   // material-point state and its converged copy, F, F_last_inc, F_dot, the
   // coordinates and the solver scalars; load() needs a context built from
@@ -149,6 +161,7 @@ class Rve {
   double *d_u = nullptr, *d_xp = nullptr, *d_xn = nullptr, *d_fsum = nullptr;  // coordinates
   double* d_n2 = nullptr;  // per-cell |dPdF|^2
   double* d_scr_partial = nullptr;  // partial sums of the iteration counters
+  const double* f_eval = nullptr;   // F of the last response, until sync_response_F
 
   // MechBase state (mech_base.h:75-107), column-major
   double C_ref[81], C_volAvg[81], C_minMaxAvg[81], C_volAvgLastInc[81], C_minMaxAvgLastInc[81], S[81];

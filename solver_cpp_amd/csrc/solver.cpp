@@ -342,7 +342,7 @@ Rve::Rve(const sc_rve_desc& d) {
   d_fsum = dalloc(9);
   d_n2 = dalloc(N);
   d_scr_partial = dalloc((size_t)2 * red_blocks());
-  SCA_HIP(hipMalloc(&d_ill, sizeof(int)));
+  d_ill = reinterpret_cast<int*>(small + SM_ILL);
   SCA_HIP(hipMalloc(&d_minmax_i, 2 * red_blocks() * sizeof(long long)));
   if (d.grid_num.derivative != 0) {
     // xi table in the kernel layout: (3 complex) per k = i + nr*(y + Ny*z)
@@ -419,7 +419,6 @@ Rve::~Rve() {
   for (double* p : allocs) (void)hipFree(p);
   if (mp.iters) (void)hipFree(mp.iters);
   if (mp.repivot) (void)hipFree(mp.repivot);
-  if (d_ill) (void)hipFree(d_ill);
   if (d_minmax_i) (void)hipFree(d_minmax_i);
   fft.release();
   if (st) (void)hipStreamDestroy(st);
@@ -444,10 +443,12 @@ std::vector<double> Rve::reduce(const std::vector<int>& ops, const std::vector<c
 // Spectral::constitutive_response (spectral.cpp:132-219) up to the P_av sums
 // on the device; the tangent reductions are deferred to tangent().
 void Rve::constitutive_response(const double* Fdev, double dt) {
-  SCA_HIP(hipMemcpyAsync(mp.F, Fdev, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
   SCA_HIP(hipMemsetAsync(d_ill, 0, sizeof(int), st));
+  MpState s = mp;
+  s.F = const_cast<double*>(Fdev);  // read only
+  f_eval = Fdev;
   kt.begin(SC_KT_RESPONSE, st);
-  launch_mp_response(mp, mc, dt, d_ill, sub, st);
+  launch_mp_response(s, mc, dt, d_ill, sub, st);
   kt.end(SC_KT_RESPONSE, N, st);
   kt.begin(SC_KT_REDUCE, st);
   launch_pav_partial(mp.P, N, partial, st);
@@ -462,16 +463,24 @@ void Rve::constitutive_response(const double* Fdev, double dt) {
   last_dt = dt;
 }
 
+void Rve::sync_response_F() {
+  if (f_eval && f_eval != mp.F)
+    SCA_HIP(hipMemcpyAsync(mp.F, f_eval, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  f_eval = mp.F;
+}
+
 void Rve::fetch_response_scalars(bool rotate) {
+  // P sums, the terminally-ill flag and the divergence sum: adjacent slots,
+  // one copy
+  static_assert(SM_ILL == SM_SUMS + 9 && SM_DIV == SM_ILL + 1, "response scalars must be adjacent");
   double h_s[11];
   int ill = 0;
-  SCA_HIP(hipMemcpyAsync(h_s, small + SM_SUMS, 9 * sizeof(double), hipMemcpyDeviceToHost, st));
-  SCA_HIP(hipMemcpyAsync(h_s + 9, small + SM_DIV, sizeof(double), hipMemcpyDeviceToHost, st));
-  SCA_HIP(hipMemcpyAsync(&ill, d_ill, sizeof(int), hipMemcpyDeviceToHost, st));
+  SCA_HIP(hipMemcpyAsync(h_s, small + SM_SUMS, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
   SCA_HIP(hipStreamSynchronize(st));
+  std::memcpy(&ill, &h_s[9], sizeof ill);
   for (int q = 0; q < 9; ++q) P_av[q] = h_s[q] * wgt;
   if (rotate) h::rot2(rot_q, P_av, P_av);
-  last_div_sum = h_s[9];
+  last_div_sum = h_s[10];
   if (ill) terminally_ill = true;
   if (kt.on) {
     double it[2];
@@ -486,6 +495,7 @@ void Rve::fetch_response_scalars(bool rotate) {
 // consistent tangent of the last evaluation: C_volAvg, C_minMaxAvg
 // (spectral.cpp:189-217)
 void Rve::tangent() {
+  sync_response_F();
   kt.begin(SC_KT_TANGENT, st);
   launch_mp_tangent(mp, mc, last_dt, dPdF, st);
   kt.end(SC_KT_TANGENT, N, st);
@@ -604,25 +614,28 @@ Ngmres::Ngmres(int64_t n_, hipStream_t st_, KTimer* kt_) : n(n_), st(st_), kt(kt
     SCA_HIP(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(double)));
     bufs.push_back(p);
   };
-  a(Fv, n);
-  a(XM, n);
-  a(FM, n);
-  a(XA, n);
-  a(FA, n);
   a(partial, (size_t)red_blocks() * MAX_RED_VEC);
   a(small, MAX_RED_VEC);
 }
 Ngmres::~Ngmres() {
   for (double* p : bufs) (void)hipFree(p);
-  for (double* p : hist) (void)hipFree(p);
+  for (double* p : pool) (void)hipFree(p);
 }
-double* Ngmres::slot(int k) {
-  while ((int)hist.size() <= k) {
-    double* p = nullptr;
-    SCA_HIP(hipMalloc(&p, n * sizeof(double)));
-    hist.push_back(p);
-  }
-  return hist[k];
+double* Ngmres::take() {
+  auto used = [&](const double* p) {
+    if (p == Fv || p == Xc || p == XM || p == FM || p == XA || p == FA) return true;
+    for (const double* q : Fh)
+      if (q == p) return true;
+    for (const double* q : Xh)
+      if (q == p) return true;
+    return false;
+  };
+  for (double* p : pool)
+    if (!used(p)) return p;
+  double* p = nullptr;
+  SCA_HIP(hipMalloc(&p, std::max<int64_t>(n, 1) * sizeof(double)));
+  pool.push_back(p);
+  return p;
 }
 std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
                                    const std::vector<const double*>& b) {
@@ -657,28 +670,45 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
   const int max_its = 50;
   const double gammaA = 2.0, gammaC = 2.0, epsilonB = 0.1, deltaB = 0.9;
   const int restart_it = 2;
-  const size_t bytes = n * sizeof(double);
-  auto Fdot = [&](int i) { return slot(2 * i); };
-  auto Xdot = [&](int i) { return slot(2 * i + 1); };
+  auto Fdot = [&](int i) { return Fh[i]; };
+  auto Xdot = [&](int i) { return Xh[i]; };
   std::vector<double> fnorms(msize, 0.0), Q(msize * msize, 0.0);
+  Fv = XM = FM = XA = FA = nullptr;
+  Fh.assign(msize, nullptr);
+  Xh.assign(msize, nullptr);
+  Xc = X;
+  // the solution into the caller's X; every reference released
+  auto done = [&](int r) {
+    if (Xc != X) SCA_HIP(hipMemcpyAsync(X, Xc, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    Fv = Xc = XM = FM = XA = FA = nullptr;
+    Fh.assign(msize, nullptr);
+    Xh.assign(msize, nullptr);
+    return r;
+  };
+  Fv = take();
   set_iter(0);
-  residual(X, Fv);
+  residual(Xc, Fv);
   double fnorm = std::sqrt(reduce({0}, {Fv}, {Fv})[0]);
-  if (std::isnan(fnorm) || std::isinf(fnorm)) return -4;  // SNES_DIVERGED_FNORM_NAN
+  if (std::isnan(fnorm) || std::isinf(fnorm)) return done(-4);  // SNES_DIVERGED_FNORM_NAN
   double fminnorm = fnorm;
   int reason = conv(0);
-  if (reason) return reason > 0 ? 6 : -5;
-  auto update_subspace = [&](int iv, const double* f, double fn, const double* x) {
-    SCA_HIP(hipMemcpyAsync(Fdot(iv), f, bytes, hipMemcpyDeviceToDevice, st));
-    SCA_HIP(hipMemcpyAsync(Xdot(iv), x, bytes, hipMemcpyDeviceToDevice, st));
+  if (reason) return done(reason > 0 ? 6 : -5);
+  // SNESNGMRESUpdateSubspace_Private: the slot references the vectors
+  auto update_subspace = [&](int iv, double* f, double fn, double* x) {
+    Fh[iv] = f;
+    Xh[iv] = x;
     fnorms[iv] = fn;
   };
-  update_subspace(0, Fv, fnorm, X);
+  update_subspace(0, Fv, fnorm, Xc);
   int k_restart = 1, l = 1, ivec = 0, restart_count = 0;
   for (int k = 1; k < max_its + 1; ++k) {
+    XM = nullptr;
+    XM = take();
     if (kt) kt->begin(SC_KT_REDUCE, st);
-    launch_axpby(1.0, X, -1.0, Fv, XM, n, st);  // basic line search: XM = X - F
+    launch_axpby(1.0, Xc, -1.0, Fv, XM, n, st);  // basic line search: XM = X - F
     if (kt) kt->end(SC_KT_REDUCE, n, st);
+    FM = nullptr;
+    FM = take();
     residual(XM, FM);
     // ||FM||^2, <FM, Fdot_i>, <Fdot_ivec, Fdot_i>: one pass over FM,
     // Fdot_ivec and the history (each vector read once)
@@ -709,7 +739,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       dots = reduce(ops, va, vb);
     }
     const double fMnorm = std::sqrt(dots[0]);
-    if (std::isnan(fMnorm) || std::isinf(fMnorm)) return -6;  // SNES_DIVERGED_LINE_SEARCH
+    if (std::isnan(fMnorm) || std::isinf(fMnorm)) return done(-6);  // SNES_DIVERGED_LINE_SEARCH
     {  // SNESNGMRESFormCombinedSolution_Private
       const double nu = fMnorm * fMnorm;
       std::vector<double> xi(l), beta(l);
@@ -731,9 +761,13 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       for (int i = 0; i < l; ++i) alph_total += beta[i];
       std::vector<const double*> xs(l);
       for (int i = 0; i < l; ++i) xs[i] = Xdot(i);
+      XA = nullptr;
+      XA = take();
       if (kt) kt->begin(SC_KT_REDUCE, st);
       launch_lincomb(XM, 1.0 - alph_total, l, beta.data(), xs.data(), XA, n, st);
       if (kt) kt->end(SC_KT_REDUCE, n * (l + 1), st);
+      FA = nullptr;
+      FA = take();
       residual(XA, FA);
     }
     if (fminnorm > fMnorm) fminnorm = fMnorm;
@@ -771,7 +805,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       const double d = std::sqrt(nrm[2 + i]);
       if (d < dminnorm || dminnorm < 0.0) dminnorm = d;
     }
-    if (std::isnan(fnorm) || std::isinf(fnorm)) return -4;
+    if (std::isnan(fnorm) || std::isinf(fnorm)) return done(-4);
     // SNESNGMRESSelect_Private / SelectRestart_Private (difference)
     bool selectA = fAnorm < gammaA * fminnorm;
     if (!(epsilonB * dnorm < dminnorm || std::sqrt(fnorm) < deltaB * std::sqrt(fminnorm))) selectA = false;
@@ -781,15 +815,18 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
     restart_count = selectRestart ? restart_count + 1 : 0;
     ivec = k_restart % msize;
     const bool do_restart = restart_count >= restart_it;
-    if (do_restart) update_subspace(0, FM, fMnorm, XM);
+    if (do_restart) {
+      update_subspace(0, FM, fMnorm, XM);
+      for (int i = 1; i < msize; ++i) Fh[i] = Xh[i] = nullptr;
+    }
     if (selectA) {
       fnorm = fAnorm;
-      SCA_HIP(hipMemcpyAsync(Fv, FA, bytes, hipMemcpyDeviceToDevice, st));
-      SCA_HIP(hipMemcpyAsync(X, XA, bytes, hipMemcpyDeviceToDevice, st));
+      Fv = FA;
+      Xc = XA;
     } else {
       fnorm = fMnorm;
-      SCA_HIP(hipMemcpyAsync(Fv, FM, bytes, hipMemcpyDeviceToDevice, st));
-      SCA_HIP(hipMemcpyAsync(X, XM, bytes, hipMemcpyDeviceToDevice, st));
+      Fv = FM;
+      Xc = XM;
     }
     if (do_restart) {
       restart_count = 0;
@@ -800,13 +837,13 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       if (l < msize) l++;
       k_restart++;
       if (fminnorm > fnorm) fminnorm = fnorm;
-      update_subspace(ivec, Fv, fnorm, X);
+      update_subspace(ivec, Fv, fnorm, Xc);
     }
     set_iter(k);
     reason = conv(k);
-    if (reason) return reason > 0 ? 6 : -5;
+    if (reason) return done(reason > 0 ? 6 : -5);
   }
-  return -5;  // SNES_DIVERGED_MAX_IT
+  return done(-5);  // SNES_DIVERGED_MAX_IT
 }
 
 int Rve::snes_ngmres() {
@@ -852,6 +889,7 @@ struct CkptHeader {
 }  // namespace
 
 void Rve::save(std::vector<char>& blob) {
+  sync_response_F();
   const std::vector<std::pair<double*, int64_t>> planes = {
       {mp.F, 9 * N},       {mp.F0, 9 * N},    {mp.Fp0, 9 * N},   {mp.Lp0, 9 * N},   {mp.S0, 9 * N},
       {mp.xi0, 12 * N},    {mp.gamma0, 12 * N}, {mp.Lp, 9 * N},  {mp.P, 9 * N},     {mp.Fp, 9 * N},
@@ -959,6 +997,7 @@ void Rve::load(const std::vector<char>& blob) {
     SCA_HIP(hipMemcpyAsync(pl.first, blob.data() + off, pl.second * sizeof(double), hipMemcpyHostToDevice, st));
     off += pl.second * sizeof(double);
   }
+  f_eval = nullptr;  // mp.F holds the restored F
   SCA_HIP(hipStreamSynchronize(st));
 }
 
@@ -997,6 +1036,7 @@ void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_r
     SCA_HIP(hipMemcpyAsync(small + SM_AV, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
     launch_rate(guess ? 1 : 0, F_last_inc, X, dt_old, small + SM_AV, F_dot, N, st);
     SCA_HIP(hipMemcpyAsync(F_last_inc, X, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    sync_response_F();
     launch_mp_forward(mp, st);  // f_materialpoint_forward
   }
   for (int q = 0; q < 9; ++q) F_aim[q] = F_aim_last_inc[q] + F_aim_dot[q] * dt;
