@@ -739,6 +739,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       dots = reduce(ops, va, vb);
     }
     const double fMnorm = std::sqrt(dots[0]);
+    bool fused = false;
     if (std::isnan(fMnorm) || std::isinf(fMnorm)) return done(-6);  // SNES_DIVERGED_LINE_SEARCH
     {  // SNESNGMRESFormCombinedSolution_Private
       const double nu = fMnorm * fMnorm;
@@ -763,8 +764,12 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       for (int i = 0; i < l; ++i) xs[i] = Xdot(i);
       XA = nullptr;
       XA = take();
+      // the distances of XA to XM and to the history (SNESNGMRESNorms_Private)
+      // come with the combination when the history fits one pass
       if (kt) kt->begin(SC_KT_REDUCE, st);
-      launch_lincomb(XM, 1.0 - alph_total, l, beta.data(), xs.data(), XA, n, st);
+      fused = launch_lincomb_norms(XM, 1.0 - alph_total, l, beta.data(), xs.data(), XA, n, partial + red_blocks(),
+                                   st);
+      if (!fused) launch_lincomb(XM, 1.0 - alph_total, l, beta.data(), xs.data(), XA, n, st);
       if (kt) kt->end(SC_KT_REDUCE, n * (l + 1), st);
       FA = nullptr;
       FA = take();
@@ -774,7 +779,13 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
     // ||FA||^2, ||XA - XM||^2, ||Xdot_i - XA||^2 (SNESNGMRESNorms_Private):
     // FA alone, then XM and the history against XA in one pass
     std::vector<double> nrm;
-    if (2 + l <= MAX_RED_VEC) {
+    if (fused) {
+      if (kt) kt->begin(SC_KT_REDUCE, st);
+      const int op0 = 0;
+      const double* fa = FA;
+      launch_partial_reduce<double>(&op0, n, 1, &fa, &fa, partial, st);
+      nrm = collect(2 + l, n);
+    } else if (2 + l <= MAX_RED_VEC) {
       if (kt) kt->begin(SC_KT_REDUCE, st);
       const int op0 = 0;
       const double* fa = FA;

@@ -78,6 +78,12 @@ void launch_rate(int het, const double* f0, const double* f, double dt, const do
 void launch_axpby(double a, const double* x, double b, const double* y0, double* y, int64_t n, hipStream_t st);
 void launch_lincomb(const double* XM, double c0, int l, const double* beta, const double* const* X, double* XA,
                     int64_t n, hipStream_t st);
+// XA as launch_lincomb plus the partials (l+1 rows, red_blocks() each) of
+// |XA - XM|^2 and |XA - X_i|^2, bitwise those of launch_shared_reduce(1, n,
+// XA, -, l+1, {XM, X_0..}); false (nothing launched) when l + 1 exceeds one
+// shared group
+bool launch_lincomb_norms(const double* XM, double c0, int l, const double* beta, const double* const* X,
+                          double* XA, int64_t n, double* partial, hipStream_t st);
 void launch_soa_to_aos(const double* soa, double* aos, int64_t N, int ncomp, hipStream_t st);
 void launch_aos_to_soa(const double* aos, double* soa, int64_t N, int ncomp, hipStream_t st);
 void launch_f64_to_f32(const double* a, float* b, int64_t n, hipStream_t st);

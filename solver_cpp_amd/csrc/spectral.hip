@@ -688,6 +688,67 @@ __global__ void lincomb_kernel(const double* XM, CombArgs a, double* XA, int64_t
   for (int i = 0; i < a.l; ++i) v += a.beta[i] * a.X[i][t];
   XA[t] = v;
 }
+// XA = XM*c0 + sum_i beta_i Xdot_i (the sum in lincomb_kernel's order) and,
+// in the same pass, the partials of |XA - XM|^2 and |XA - Xdot_i|^2 (the
+// N-GMRES distances, shared_reduce_kernel<1> with u = XA, v = {XM, Xdot_i}):
+// the history is read once instead of twice.  Element order per thread and
+// the block tree are those of the shared reduction, so the distances are
+// bitwise its results; l + 1 <= SHARED_GROUP.
+struct CombNormArgs {
+  int64_t n;
+  int l;
+  double c0;
+  const double* XM;
+  double* XA;
+  double beta[SHARED_GROUP];
+  const double* X[SHARED_GROUP];
+};
+__global__ void __launch_bounds__(256) lincomb_norms_kernel(CombNormArgs r, double* partial) {
+  constexpr int CH = 8;
+  __shared__ double red[CH][256];
+  const int t = threadIdx.x;
+  double s[SHARED_GROUP];
+#pragma unroll
+  for (int j = 0; j < SHARED_GROUP; ++j) s[j] = 0.0;
+  constexpr int64_t STRIDE = (int64_t)RED_BLOCKS * 256;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < r.n; e += STRIDE) {
+    const double xm = r.XM[e];
+    double h[SHARED_GROUP - 1];
+#pragma unroll
+    for (int i = 0; i < SHARED_GROUP - 1; ++i) h[i] = (i < r.l) ? r.X[i][e] : 0.0;
+    double v = xm * r.c0;
+#pragma unroll
+    for (int i = 0; i < SHARED_GROUP - 1; ++i)
+      if (i < r.l) v += r.beta[i] * h[i];
+    r.XA[e] = v;
+    const double d0 = xm - v;
+    s[0] += d0 * d0;
+#pragma unroll
+    for (int i = 0; i < SHARED_GROUP - 1; ++i)
+      if (i < r.l) {
+        const double d = h[i] - v;
+        s[1 + i] += d * d;
+      }
+  }
+  const int nout = r.l + 1;
+#pragma unroll
+  for (int c0 = 0; c0 < SHARED_GROUP; c0 += CH) {
+    if (c0 >= nout) break;
+#pragma unroll
+    for (int k = 0; k < CH; ++k) red[k][t] = s[c0 + k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) {
+#pragma unroll
+        for (int k = 0; k < CH; ++k) red[k][t] += red[k][t + w];
+      }
+      __syncthreads();
+    }
+    if (t < CH && c0 + t < nout) partial[(int64_t)(c0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
+    __syncthreads();
+  }
+}
+
 // SoA (9 planes of N, component q = i+3j) <-> AoS (3,3,N) column-major
 __global__ void soa_to_aos_kernel(const double* soa, double* aos, int64_t N, int ncomp) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1209,6 +1270,23 @@ void launch_lincomb(const double* XM, double c0, int l, const double* beta, cons
   }
   hipLaunchKernelGGL(lincomb_kernel, dim3(nblk(n)), dim3(256), 0, st, XM, a, XA, n);
   SCA_HIP(hipGetLastError());
+}
+bool launch_lincomb_norms(const double* XM, double c0, int l, const double* beta, const double* const* X,
+                          double* XA, int64_t n, double* partial, hipStream_t st) {
+  if (l + 1 > SHARED_GROUP) return false;
+  CombNormArgs a{};
+  a.n = n;
+  a.l = l;
+  a.c0 = c0;
+  a.XM = XM;
+  a.XA = XA;
+  for (int i = 0; i < SHARED_GROUP; ++i) {
+    a.beta[i] = i < l ? beta[i] : 0.0;
+    a.X[i] = i < l ? X[i] : XM;  // unused slots stay dereferenceable
+  }
+  hipLaunchKernelGGL(lincomb_norms_kernel, dim3(RED_BLOCKS), dim3(256), 0, st, a, partial);
+  SCA_HIP(hipGetLastError());
+  return true;
 }
 void launch_soa_to_aos(const double* soa, double* aos, int64_t N, int ncomp, hipStream_t st) {
   hipLaunchKernelGGL(soa_to_aos_kernel, dim3(nblk(ncomp * N)), dim3(256), 0, st, soa, aos, N, ncomp);
