@@ -437,9 +437,12 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
                                                 const PointIO& io, const MatConst& mc) {
   const int64_t N = io.N;
   const Lane l = io.l;
-  double A[9];
+  // A is read from LDS where it is used (volatile: not hoisted into the
+  // registers the Newton system needs)
+  auto loadA = [&](double* A) {
 #pragma unroll
-  for (int k = 0; k < 9; ++k) A[k] = hA[k][t];
+    for (int k = 0; k < 9; ++k) A[k] = ((const volatile double*)hA[k])[t];
+  };
   double Lpg_old[9], dLp[9];
   double res_old_n2 = 0.0;  // squared norms: the tests compare non-negative values
   double step = 1.0;
@@ -460,9 +463,10 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
     }
     double Fe[9], Sl[9];
     {
-      double B[9];
+      double B[9], A[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) B[k] = (k % 4 == 0 ? 1.0 : 0.0) - dt * Lp[k];
+      loadA(A);
       mm33(A, B, Fe);  // Fe = A B invFi_new, invFi_new = I
     }
     hooke_S(Fe, Sl, mc);
@@ -509,11 +513,13 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
     }
     bool solved = false;
     if (!redo) {
-      double J[NR * NR];
+      double J[NR * NR], A[9];
+      loadA(A);
       assemble_jacobian8(Fe, A, c, dt, J, mc);
       solved = solve8_nopivot(J, res);
     } else if constexpr (PIVOT) {  // rare: pivoted factorisation on memory operands
-      double Jm[NR * NR], bm[NR];
+      double Jm[NR * NR], bm[NR], A[9];
+      loadA(A);
       assemble_jacobian8(Fe, A, c, dt, Jm, mc);
 #pragma unroll
       for (int k = 0; k < NR; ++k) bm[k] = res[k];
@@ -547,7 +553,8 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
     mm33(iF0, B, invFp_new);
   }
   if (!inv33(invFp_new, Fpn, nullptr)) return 1;
-  double T1[9];
+  double T1[9], A[9];
+  loadA(A);
   mm33(A, B, T1);  // F invFp_new (= Fe, invFi_new = I)
   hooke_S(T1, S, mc);
   return 0;
