@@ -241,14 +241,15 @@ __device__ __forceinline__ void dot_state(const double* S, const double* st, con
 // exactly; this is the same linear solve with an 8x8 factorisation.
 constexpr int NR = 8;
 
-// Solve J y = b (8x8, row-major) by LU with partial pivoting on memory
-// operands (dynamic indices).  Kept out of line: it runs only when the LU
-// without row exchanges refuses a pivot, and must not share registers with
-// the hot path.
-__device__ __noinline__ bool solve8_pivot(double* J, double* b) {
+// Solve J y = b (8x8, row-major) by LU with partial pivoting (first largest
+// entry of the column, as LAPACK's idamax).  Used only by the pivoting pass
+// (its own kernel), with J in registers.
+__device__ __forceinline__ bool solve8_pivot(double* J, double* b) {
+#pragma unroll
   for (int k = 0; k < NR; ++k) {
     int p = k;
     double mx = fabs(J[NR * k + k]);
+#pragma unroll
     for (int i = k + 1; i < NR; ++i) {
       const double v = fabs(J[NR * i + k]);
       if (v > mx) {
@@ -257,25 +258,34 @@ __device__ __noinline__ bool solve8_pivot(double* J, double* b) {
       }
     }
     if (!(mx > 0.0)) return false;
-    if (p != k) {
-      for (int c = 0; c < NR; ++c) {
-        const double t = J[NR * k + c];
-        J[NR * k + c] = J[NR * p + c];
-        J[NR * p + c] = t;
+    // exchange rows k and p by selects over the candidate rows (static
+    // register indices; the columns left of k are not read again)
+#pragma unroll
+    for (int i = k + 1; i < NR; ++i) {
+      const bool sw = p == i;
+#pragma unroll
+      for (int c = k; c < NR; ++c) {
+        const double u = J[NR * k + c], w = J[NR * i + c];
+        J[NR * k + c] = sw ? w : u;
+        J[NR * i + c] = sw ? u : w;
       }
-      const double t = b[k];
-      b[k] = b[p];
-      b[p] = t;
+      const double u = b[k], w = b[i];
+      b[k] = sw ? w : u;
+      b[i] = sw ? u : w;
     }
     const double inv = 1.0 / J[NR * k + k];
+#pragma unroll
     for (int i = k + 1; i < NR; ++i) {
       const double l = J[NR * i + k] * inv;
+#pragma unroll
       for (int c = k + 1; c < NR; ++c) J[NR * i + c] -= l * J[NR * k + c];
       b[i] -= l * b[k];
     }
   }
+#pragma unroll
   for (int i = NR - 1; i >= 0; --i) {
     double s = b[i];
+#pragma unroll
     for (int c = i + 1; c < NR; ++c) s -= J[NR * i + c] * b[c];
     b[i] = s / J[NR * i + i];
   }
@@ -423,7 +433,8 @@ struct PointIO {
 
 // integrateStress (phase_mechanical.f90:380-604); returns 0 (integrated),
 // 1 (broken) or, without PIVOT, 2 when the LU without row exchanges refused a
-// pivot (the point is then redone by the PIVOT instantiation).  Uses the
+// pivot (the point is then redone by the PIVOT instantiation, which solves
+// every Newton system with partial pivoting).  Uses the
 // state xi, updates S and Lp in registers and writes P, Fp, Fe of a
 // successful integration straight to memory (they are read again only as
 // results or as the start of a following substep).
@@ -452,15 +463,12 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
     dLp[k] = 0.0;
   }
   int it = 0;
-  bool redo = false;  // repeat this iteration's solve with row pivoting
   // B, Fe and S of an iteration are not carried across the linear solve: the
   // converged ones are formed again from the converged Lp after the loop (S
   // is an output only on success)
   for (;;) {
-    if (!redo) {
-      it++;
-      if (it > mc.N_iter_Lp_max) return 1;
-    }
+    it++;
+    if (it > mc.N_iter_Lp_max) return 1;
     double Fe[9], Sl[9];
     {
       double B[9], A[9];
@@ -491,7 +499,7 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
       res[k] = Lp[k] - res[k];
       n2res += res[k] * res[k];
     }
-    if (!redo) {
+    {
       // |res| < max(eps_rel max(|Lp|, |Lp_c|), eps_abs), compared squared
       const double atol = fmax(mc.eps_rel_Lp * sqrt(fmax(n2Lp, n2Lpc)), mc.eps_abs_Lp);
       bool nan = false;
@@ -511,29 +519,16 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
         continue;
       }
     }
-    bool solved = false;
-    if (!redo) {
+    {
       double J[NR * NR], A[9];
       loadA(A);
       assemble_jacobian8(Fe, A, c, dt, J, mc);
-      solved = solve8_nopivot(J, res);
-    } else if constexpr (PIVOT) {  // rare: pivoted factorisation on memory operands
-      double Jm[NR * NR], bm[NR], A[9];
-      loadA(A);
-      assemble_jacobian8(Fe, A, c, dt, Jm, mc);
-#pragma unroll
-      for (int k = 0; k < NR; ++k) bm[k] = res[k];
-      solved = solve8_pivot(Jm, bm);
-#pragma unroll
-      for (int k = 0; k < NR; ++k) res[k] = bm[k];
+      if constexpr (PIVOT) {
+        if (!solve8_pivot(J, res)) return 1;  // singular
+      } else {
+        if (!solve8_nopivot(J, res)) return 2;  // the point is redone by the pivoting pass
+      }
     }
-    if (!solved) {
-      if constexpr (!PIVOT) return 2;  // the point is redone by the pivoting pass
-      if (redo) return 1;
-      redo = true;
-      continue;
-    }
-    redo = false;
     res[8] = -(res[0] + res[4]);  // x_22 of the traceless solution
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -614,7 +609,8 @@ constexpr int SUB_F0 = 0, SUB_LP0 = 9, SUB_FP0 = 18, SUB_ST0 = 27, SUB_FLOC = 51
 // appended to the repivot list (its inputs are unchanged: Lp, the only plane
 // that is both read and written, is stored last).  The second
 // (mp_repivot_kernel) redoes exactly the listed points from the same inputs,
-// with the pivoted fallback, in a grid-stride loop over the list.  Points are
+// every Newton system of them solved with partial pivoting (as LAPACK's
+// dgesv in the reference), in a grid-stride loop over the list.  Points are
 // independent, so the list order does not change any result.
 //
 // repivot: [0] = number of listed points, [1 + i] = cell index.
