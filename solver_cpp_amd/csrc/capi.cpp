@@ -433,6 +433,7 @@ int sc_rve_form_residual(sc_rve* R, const double* F, double* r) {
     a.put(F, v.st);
     sca::launch_aos_to_soa(a.p, soa.p, N, 9, v.st);
     v.form_residual(soa.p, res.p);
+    v.complete_response_scalars();
     v.sync_response_F();
     sca::launch_soa_to_aos(res.p, a.p, N, 9, v.st);
     a.get(r, v.st);

@@ -29,7 +29,7 @@ enum {
   SM_AIM2 = 140,  // 9: forward_field aim
   SM_FSUM = 150,  // 9: forward_field sums
   SM_FDIFF = 160, // 9: forward_field correction
-  SM_ITERS = 180, // 2: Newton / state iterations of the last response (profiling)
+  SM_ITERS = 111, // 2: Newton / state iterations of the last response (profiling)
   SM_G81 = 200    // 162: gathered dPdF of argmax / argmin
 };
 
@@ -68,6 +68,9 @@ class Ngmres {
                              const std::vector<const double*>& b);
   // final sums of the first nout partial rows, to the host
   std::vector<double> collect(int nout, int64_t units);
+  // called after every synchronisation of the reductions (the owner takes up
+  // results it copied asynchronously)
+  std::function<void()> after_sync;
   int64_t n;
   hipStream_t st;
   double *partial = nullptr, *small = nullptr;
@@ -109,6 +112,8 @@ class Rve {
   void form_residual(const double* Fdev, double* rdev);
   void constitutive_response(const double* Fdev, double dt);
   void fetch_response_scalars(bool rotate);
+  void request_response_scalars(bool rotate, bool residual);
+  void complete_response_scalars();  // no-op unless a request is pending
   void tangent();
   void update_coords();
   // mp.F <- the F of the last constitutive response (the response reads F
@@ -162,6 +167,8 @@ class Rve {
   double* d_n2 = nullptr;  // per-cell |dPdF|^2
   double* d_scr_partial = nullptr;  // partial sums of the iteration counters
   const double* f_eval = nullptr;   // F of the last response, until sync_response_F
+  double* h_scal = nullptr;  // pinned: response scalars in flight
+  bool scal_pending = false, scal_rotate = false, scal_residual = false;
 
   // MechBase state (mech_base.h:75-107), column-major
   double C_ref[81], C_volAvg[81], C_minMaxAvg[81], C_volAvgLastInc[81], C_minMaxAvgLastInc[81], S[81];

@@ -304,6 +304,8 @@ Rve::Rve(const sc_rve_desc& d) {
   SCA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   fft.init(cells);
   ngm = new Ngmres(9 * N, st, &kt);
+  ngm->after_sync = [this] { complete_response_scalars(); };
+  SCA_HIP(hipHostMalloc((void**)&h_scal, 16 * sizeof(double), hipHostMallocDefault));
   mixed = d.precision == SC_PRECISION_MIXED;
   if (mixed) {
     fft32.init(cells);
@@ -414,6 +416,7 @@ Rve::Rve(const sc_rve_desc& d) {
 
 Rve::~Rve() {
   delete ngm;
+  if (h_scal) (void)hipHostFree(h_scal);
   if (spec32) (void)hipFree(spec32);
   fft32.release();
   for (double* p : allocs) (void)hipFree(p);
@@ -469,32 +472,57 @@ void Rve::sync_response_F() {
   f_eval = mp.F;
 }
 
-void Rve::fetch_response_scalars(bool rotate) {
-  // P sums, the terminally-ill flag and the divergence sum: adjacent slots,
-  // one copy
-  static_assert(SM_ILL == SM_SUMS + 9 && SM_DIV == SM_ILL + 1, "response scalars must be adjacent");
-  double h_s[11];
-  int ill = 0;
-  SCA_HIP(hipMemcpyAsync(h_s, small + SM_SUMS, 11 * sizeof(double), hipMemcpyDeviceToHost, st));
+// The response scalars (P sums, the terminally-ill flag, the divergence sum
+// and, profiling, the iteration counts: adjacent slots) travel in one
+// asynchronous copy to pinned memory; they are taken up at the next stream
+// synchronisation the host makes anyway (the N-GMRES reductions), so an
+// evaluation costs no round trip of its own.
+void Rve::request_response_scalars(bool rotate, bool residual) {
+  static_assert(SM_ILL == SM_SUMS + 9 && SM_DIV == SM_ILL + 1 && SM_ITERS == SM_DIV + 1,
+                "response scalars must be adjacent");
+  SCA_HIP(hipMemcpyAsync(h_scal, small + SM_SUMS, (kt.on ? 13 : 11) * sizeof(double), hipMemcpyDeviceToHost, st));
+  scal_pending = true;
+  scal_rotate = rotate;
+  scal_residual = residual;
+}
+
+void Rve::complete_response_scalars() {
+  if (!scal_pending) return;
   SCA_HIP(hipStreamSynchronize(st));
-  std::memcpy(&ill, &h_s[9], sizeof ill);
-  for (int q = 0; q < 9; ++q) P_av[q] = h_s[q] * wgt;
-  if (rotate) h::rot2(rot_q, P_av, P_av);
-  last_div_sum = h_s[10];
+  scal_pending = false;
+  int ill = 0;
+  std::memcpy(&ill, &h_scal[9], sizeof ill);
+  for (int q = 0; q < 9; ++q) P_av[q] = h_scal[q] * wgt;
+  if (scal_rotate) h::rot2(rot_q, P_av, P_av);
+  last_div_sum = h_scal[10];
   if (ill) terminally_ill = true;
   if (kt.on) {
-    double it[2];
-    SCA_HIP(hipMemcpyAsync(it, small + SM_ITERS, sizeof it, hipMemcpyDeviceToHost, st));
-    SCA_HIP(hipStreamSynchronize(st));
-    stats.newton_iterations += (int64_t)it[0];
-    stats.state_iterations += (int64_t)it[1];
+    stats.newton_iterations += (int64_t)h_scal[11];
+    stats.state_iterations += (int64_t)h_scal[12];
   }
   kt.flush();
+  if (!scal_residual) return;
+  // the host part of form_residual (mech_solver_basic.cpp:286-339)
+  err_div = std::sqrt(last_div_sum) * wgt;
+  if (cells[0] == 1) err_div = err_div / 2;
+  double dF[9];
+  h::mul3333xx33(S, P_av, dF);
+  for (int q = 0; q < 9; ++q) F_aim[q] = F_aim[q] - dF[q];
+  double e = 0.0;
+  for (int q = 0; q < 9; ++q)
+    if (!stress_mask[q]) e = std::max(e, std::fabs(P_av[q] - P_aim[q]));
+  err_BC = e;
+}
+
+void Rve::fetch_response_scalars(bool rotate) {
+  request_response_scalars(rotate, false);
+  complete_response_scalars();
 }
 
 // consistent tangent of the last evaluation: C_volAvg, C_minMaxAvg
 // (spectral.cpp:189-217)
 void Rve::tangent() {
+  complete_response_scalars();
   sync_response_F();
   kt.begin(SC_KT_TANGENT, st);
   launch_mp_tangent(mp, mc, last_dt, dPdF, st);
@@ -578,16 +606,9 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
     kt.end(SC_KT_FFT_BWD, N, st);
     evals64++;
   }
-  fetch_response_scalars(true);
-  err_div = std::sqrt(last_div_sum) * wgt;
-  if (cells[0] == 1) err_div = err_div / 2;
-  double dF[9];
-  h::mul3333xx33(S, P_av, dF);
-  for (int q = 0; q < 9; ++q) F_aim[q] = F_aim[q] - dF[q];
-  double e = 0.0;
-  for (int q = 0; q < 9; ++q)
-    if (!stress_mask[q]) e = std::max(e, std::fabs(P_av[q] - P_aim[q]));
-  err_BC = e;
+  // P_av, err_div, F_aim and err_BC of this evaluation are completed at the
+  // next synchronisation (complete_response_scalars)
+  request_response_scalars(true, true);
   nfuncs++;
   solve_evals++;
   stats.residual_evaluations++;
@@ -651,6 +672,7 @@ std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vecto
     SCA_HIP(hipMemcpyAsync(out.data() + v0, small, nv * sizeof(double), hipMemcpyDeviceToHost, st));
     SCA_HIP(hipStreamSynchronize(st));
   }
+  if (after_sync) after_sync();
   if (kt) kt->flush();
   return out;
 }
@@ -661,6 +683,7 @@ std::vector<double> Ngmres::collect(int nout, int64_t units) {
   if (kt) kt->end(SC_KT_REDUCE, units, st);
   SCA_HIP(hipMemcpyAsync(out.data(), small, nout * sizeof(double), hipMemcpyDeviceToHost, st));
   SCA_HIP(hipStreamSynchronize(st));
+  if (after_sync) after_sync();
   if (kt) kt->flush();
   return out;
 }
@@ -900,6 +923,7 @@ struct CkptHeader {
 }  // namespace
 
 void Rve::save(std::vector<char>& blob) {
+  complete_response_scalars();
   sync_response_F();
   const std::vector<std::pair<double*, int64_t>> planes = {
       {mp.F, 9 * N},       {mp.F0, 9 * N},    {mp.Fp0, 9 * N},   {mp.Lp0, 9 * N},   {mp.S0, 9 * N},
