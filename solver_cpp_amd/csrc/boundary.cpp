@@ -255,7 +255,6 @@ void f_homogenization_mechanical_response(double* dt, int* cell_start, int* cell
         SCA_HIP(hipStreamSynchronize(b.st));
       }
     }
-    SCA_HIP(hipMemsetAsync(b.d_ill, 0, sizeof(int), b.st));
     launch_mp_response(b.mp, b.mc, *dt, b.d_ill, b.sub, b.st, nullptr, c0, c1);
     int ill = 0;
     SCA_HIP(hipMemcpyAsync(&ill, b.d_ill, sizeof(int), hipMemcpyDeviceToHost, b.st));

@@ -861,7 +861,6 @@ int sc_mp_update(int64_t n, double dt, const sc_material* m, const sc_phase_nume
     up(Lp, st.Lp, 9);
     up(xi0, st.xi0, 12);
     up(gamma0, st.gamma0, 12);
-    SCA_HIP(hipMemsetAsync(d_ill, 0, sizeof(int), s.s));
     sca::launch_mp_response(st, mc, dt, d_ill, sub.p, s.s, d_br);
     down(st.Lp, Lp, 9);
     down(st.P, P, 9);

@@ -70,7 +70,8 @@ struct MpState {
 };
 
 // Kernel launchers (constitutive.hip)
-// cells [c_begin, c_end) of the planes (c_end < 0: all N)
+// cells [c_begin, c_end) of the planes (c_end < 0: all N); *d_ill is cleared
+// by the launch and set when a point is terminally ill
 void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_ill, double* d_sub, hipStream_t st,
                         int* d_broken = nullptr, int64_t c_begin = 0, int64_t c_end = -1);
 int mp_partial_blocks(int64_t N);

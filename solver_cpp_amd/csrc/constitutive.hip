@@ -860,24 +860,44 @@ __global__ void __launch_bounds__(64) mp_repivot_kernel(KernelArgs a) {
     mp_point<NE4, AE4, true>(a, a.repivot[1 + i], sh);
 }
 
-__global__ void repivot_reset_kernel(int* repivot) { repivot[0] = 0; }
+// clears the repivot list and the terminally-ill flag of one response
+__global__ void repivot_reset_kernel(int* repivot, int* ill) {
+  repivot[0] = 0;
+  *ill = 0;
+}
 
 
-// Block partial sums of P for P_av in a fixed-order tree (deterministic):
-// block b sums cells [b*256, b*256+256) of each of the 9 planes.
+// Block partial sums of P for P_av (deterministic): block b sums cells
+// [b*PAV_CELLS, (b+1)*PAV_CELLS) of each of the 9 planes, every thread its
+// PAV_CELLS/256 cells at stride 256 in order, then one fixed tree over the
+// 256 threads for all 9 sums at once.
+constexpr int PAV_CELLS = 2048;
 __global__ void __launch_bounds__(256) pav_partial_kernel(const double* P, int64_t N, double* partial) {
-  __shared__ double red[256];
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  for (int q = 0; q < 9; ++q) {
-    red[threadIdx.x] = (c < N) ? P[q * N + c] : 0.0;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
+  __shared__ double red[9][256];
+  const int t = threadIdx.x;
+  const int64_t c0 = (int64_t)blockIdx.x * PAV_CELLS + t;
+  double acc[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+#pragma unroll
+  for (int u = 0; u < PAV_CELLS / 256; ++u) {
+    const int64_t c = c0 + (int64_t)u * 256;
+    if (c < N) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] += P[q * N + c];
     }
-    if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = red[0];
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) red[q][t] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) red[q][t] += red[q][t + w];
+    }
     __syncthreads();
   }
+  if (t < 9) partial[(int64_t)t * gridDim.x + blockIdx.x] = red[t][0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1177,7 +1197,7 @@ __global__ void mp_init_kernel(MpState s, const double* Fp_init, double xi_0) {
 }
 
 // ---- launchers ---------------------------------------------------------------
-int mp_partial_blocks(int64_t N) { return (int)((N + 255) / 256); }
+int mp_partial_blocks(int64_t N) { return (int)((N + PAV_CELLS - 1) / PAV_CELLS); }
 int mp_sub_planes() { return SUB_PLANES; }
 
 void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_ill, double* d_sub, hipStream_t st,
@@ -1195,7 +1215,7 @@ void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_
   // second pass: a fixed grid striding over the list (it holds at most the
   // range's points)
   const int blocks2 = std::min(blocks, 1024);
-  hipLaunchKernelGGL(repivot_reset_kernel, dim3(1), dim3(1), 0, st, s.repivot);
+  hipLaunchKernelGGL(repivot_reset_kernel, dim3(1), dim3(1), 0, st, s.repivot, d_ill);
   if (mc.n_e4 == 80 && mc.a_e4 == 9) {
     hipLaunchKernelGGL((mp_response_kernel<80, 9>), dim3(blocks), dim3(64), 0, st, a);
     hipLaunchKernelGGL((mp_repivot_kernel<80, 9>), dim3(blocks2), dim3(64), 0, st, a);

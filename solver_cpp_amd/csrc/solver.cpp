@@ -446,7 +446,6 @@ std::vector<double> Rve::reduce(const std::vector<int>& ops, const std::vector<c
 // Spectral::constitutive_response (spectral.cpp:132-219) up to the P_av sums
 // on the device; the tangent reductions are deferred to tangent().
 void Rve::constitutive_response(const double* Fdev, double dt) {
-  SCA_HIP(hipMemsetAsync(d_ill, 0, sizeof(int), st));
   MpState s = mp;
   s.F = const_cast<double*>(Fdev);  // read only
   f_eval = Fdev;
