@@ -212,8 +212,13 @@ def _pair(product, oracle, cells, grains, seed=1, ori_seed=None, **kw):
     return Rve(product, cells, size, mat, ori, **kw), Rve(oracle, cells, size, mat, ori, **kw)
 
 
-def test_init_and_single_residual(product, oracle):
-    a, b = _pair(product, oracle, (8, 6, 4), 5)
+@pytest.mark.parametrize("cells", [(8, 6, 4), (8, 6, 16), (7, 5, 12), (9, 8, 32), (6, 4, 64)])
+def test_init_and_single_residual(product, oracle, cells):
+    """Init, a solution and one residual at a perturbed F, over grids with odd
+    and even x lengths and z lengths that take the pencil FFT (12, 16, 32,
+    64) or the generic passes (4)."""
+    print(f"single residual: cells {cells}", flush=True)
+    a, b = _pair(product, oracle, cells, 5)
     a.init()
     b.init()
     sa, sb = a.scalars(), b.scalars()
