@@ -263,6 +263,28 @@ def test_loadcase_16cubed_20grains(product, oracle):
     close(a.field("x_p"), b.field("x_p"), 1e-12, "x_p")
 
 
+def test_loadcase_64cubed_200grains_first_increments(product, oracle):
+    """The drop-in configuration (64^3 grid, 200 Voronoi grains, tension to
+    2 % in 40 increments), its first two increments: identical iterations
+    and evaluations, per-point F, P, Fp and xi within 1e-10 of the oracle.
+    (Two increments keep the oracle's share to about half a minute.)"""
+    cfg = CONFIGS["C2"]
+    e = cfg["edge"]
+    a, b = _pair(product, oracle, (e, e, e), cfg["grains"], cfg["seed"], cfg["ori_seed"], **EXAMPLE_NUMERICS)
+    a.init()
+    b.init()
+    load = [tension_x(t=cfg["t"], N=cfg["N"])]
+    for ra, rb in zip(a.increments(load), b.increments(load)):
+        print(f"64^3 increment {ra['increment']}: iterations {ra['iterations']} / {rb['iterations']}", flush=True)
+        assert (ra["iterations"], ra["evaluations"]) == (rb["iterations"], rb["evaluations"])
+        if ra["increment"] == 2:
+            break
+    close(a.field("F"), b.field("F"), REL, "F")
+    close(a.field("P"), b.field("P"), REL, "P")
+    close(a.field("Fp"), b.field("Fp"), REL, "Fp")
+    close(a.field("xi"), b.field("xi"), REL, "xi")
+
+
 def test_mixed_precision_16cubed(product, oracle):
     """Mixed form on the CPU-runnable configuration: FP32 transforms and Gamma
     (FP64 material point, FP64 norms) until the convergence criteria hold,
