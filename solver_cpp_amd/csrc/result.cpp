@@ -5,7 +5,9 @@
 #include <atomic>
 #include <cstdio>
 #include <ctime>
+#include <exception>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <thread>
@@ -66,22 +68,33 @@ std::string command_line() {
 
 const char* CREATOR = "solver_cpp_amd (DADF5 1.0)";
 
+// fn(c) for c in [0, n) on up to 16 threads; the first exception thrown by
+// any of them is rethrown here after all have stopped
 template <typename Fn>
 void parallel_for(int64_t n, Fn&& fn) {
   const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
                                                              n / 4096 + 1}));
   std::vector<std::thread> pool;
   std::atomic<int64_t> next{0};
+  std::mutex mu;
+  std::exception_ptr err;
   const int64_t block = 4096;
   auto work = [&] {
-    for (int64_t b; (b = next.fetch_add(block)) < n;) {
-      const int64_t e = std::min(n, b + block);
-      for (int64_t c = b; c < e; ++c) fn(c);
+    try {
+      for (int64_t b; (b = next.fetch_add(block)) < n;) {
+        const int64_t e = std::min(n, b + block);
+        for (int64_t c = b; c < e; ++c) fn(c);
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> lock(mu);
+      if (!err) err = std::current_exception();
+      next = n;
     }
   };
   for (int64_t t = 1; t < nt; ++t) pool.emplace_back(work);
   work();
   for (auto& t : pool) t.join();
+  if (err) std::rethrow_exception(err);
 }
 
 std::vector<double> download(const double* d, int64_t n, hipStream_t st) {
