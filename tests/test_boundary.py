@@ -127,3 +127,35 @@ def test_boundary_increments_match_oracle(oracle, tmp_path):
     assert np.array_equal(P[:10], P_before[:10]) and np.array_equal(P[40:], P_before[40:])
     assert not np.array_equal(P[10:40], P_before[10:40])
     lib.sc_boundary_reset()
+
+
+def test_boundary_default_job_file_name(tmp_path, monkeypatch):
+    """Without sc_boundary_set_result_file, initDamask creates the job file
+    the reference names from its inputs (CLI.f90:311-333, result.f90:87):
+    stem(grid)_stem(load)_stem(material)_stem(numerics).hdf5 in the working
+    directory, with the setup texts and cell mappings of result_init."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from hdf5_read import H5File
+    lib = Library(PRODUCT_SO, boundary=True)
+    lib.sc_boundary_reset()
+    monkeypatch.chdir(tmp_path)
+    cells, size, mat0 = _geometry(lib)
+    N = int(np.prod(cells))
+    mpath = os.path.join(GOLDEN, "material.yaml").encode()
+    npath = os.path.join(GOLDEN, "numerics.yaml").encode()
+    load, grid = b"/some/dir/tensionX.yaml", b"../grids/grid16.v2.vti"
+    lib.f_materialpoint_initBase(mpath, len(mpath), load, len(load), grid, len(grid), npath, len(npath))
+    mat1 = (mat0 + 1).astype(np.int32)
+    nn = int(np.prod(cells + 1))
+    ip0, node0 = np.zeros((N, 3)), np.zeros((nn, 3))
+    lib.f_discretization_init(ptr(mat1, IP), C.byref(C.c_int(N)), ptr(ip0), C.byref(C.c_int(N)), ptr(node0),
+                              C.byref(C.c_int(nn)), C.byref(C.c_int(nn)))
+    lib.f_materialpoint_initDamask()
+    lib.sc_boundary_reset()  # closes the file
+    job = tmp_path / "grid16.v2_tensionX_material_numerics.hdf5"
+    assert job.exists(), sorted(p.name for p in tmp_path.iterdir())
+    h5 = H5File(str(job))
+    assert h5.keys("/") == ["cell_to", "setup"]
+    assert h5.read("/setup/material.yaml")[0].decode() == open(mpath.decode()).read()
+    assert h5.read("/setup/numerics.yaml")[0].decode() == open(npath.decode()).read()
