@@ -2,7 +2,8 @@
 // spectral.hip after cpx<T> and its helpers).
 //
 // A pencil of N points is held by TP = N / E threads, E = 8 points each
-// (E = 12 for N = 3 * 2^a).  Thread t holds the points t + TP * j (j < E).
+// (E = 12 for N = 3 * 2^a; E = 16 for single-precision 2^a pencils of 32
+// points or more).  Thread t holds the points t + TP * j (j < E).
 // Every Stockham stage (radix R dividing E, span NS) is E / R butterflies per
 // thread in registers; its butterfly b = t + TP * m reads the points
 // b + q * N / R, which are exactly the thread's registers m + q * E / R.  A
@@ -31,6 +32,13 @@ constexpr int points_per_thread(int n) {
   int m = n;
   while (m % 2 == 0) m /= 2;
   return m == 1 ? 8 : 12;
+}
+// single precision: 2^a pencils of 32 points or more hold 16 points per
+// thread, so that a thread moves as many bytes as in double precision
+template <typename T>
+constexpr int points_per_thread_t(int n) {
+  const int e = points_per_thread(n);
+  return (sizeof(T) == 4 && e == 8 && n >= 32) ? 16 : e;
 }
 constexpr int pick_radix(int rem, int E) {
   for (int r : {12, 8, 6, 4, 3, 2})
@@ -176,9 +184,9 @@ __device__ __forceinline__ void stages(cpx<T>* v, int t, cpx<T>* lds, const cpx<
   }
 }
 
-template <int N>
+template <typename T, int N>
 struct Shape {
-  static constexpr int E = points_per_thread(N);
+  static constexpr int E = points_per_thread_t<T>(N);
   static constexpr int TP = N / E;     // threads per pencil
   static constexpr int P = 256 / TP;   // pencils per 256-thread block
   static constexpr int LD = N + 1;     // padded LDS row
@@ -197,7 +205,7 @@ __device__ __forceinline__ const cpx<T>* stage_tw(cpx<T>* lds_tw, const cpx<T>* 
 template <typename T, int N, int SIGN>
 __global__ void __launch_bounds__(256) strided_kernel(cpx<T>* data, int ni, int d1, int64_t s1, int64_t s2,
                                                       int64_t es, const cpx<T>* tw_g) {
-  using S = Shape<N>;
+  using S = Shape<T, N>;
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
   const cpx<T>* tw = stage_tw<T, N>(lds + S::P * S::LD, tw_g);
@@ -221,7 +229,7 @@ __global__ void __launch_bounds__(256) strided_kernel(cpx<T>* data, int ni, int 
 // X_b[k] = (Z[k] - conj Z[N-k]) / (2 i)
 template <typename T, typename TI, int N>
 __global__ void __launch_bounds__(256) x_r2c_kernel(const TI* in, cpx<T>* out, int64_t nrows, const cpx<T>* tw_g) {
-  using S = Shape<N>;
+  using S = Shape<T, N>;
   constexpr int NR = N / 2 + 1;
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -254,7 +262,7 @@ __global__ void __launch_bounds__(256) x_r2c_kernel(const TI* in, cpx<T>* out, i
 template <typename T, typename TO, int N>
 __global__ void __launch_bounds__(256) x_c2r_kernel(const cpx<T>* in, TO* out, int64_t nrows, const cpx<T>* tw_g,
                                                     T scale) {
-  using S = Shape<N>;
+  using S = Shape<T, N>;
   constexpr int NR = N / 2 + 1;
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);

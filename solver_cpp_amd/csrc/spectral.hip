@@ -809,7 +809,7 @@ static FftPassDesc make_desc(int n, int sign) {
 
 template <typename T, int N>
 static size_t pencil_lds() {
-  return (size_t)pencil::Shape<N>::LDS * sizeof(cpx<T>);
+  return (size_t)pencil::Shape<T, N>::LDS * sizeof(cpx<T>);
 }
 
 template <typename T, typename TI>
@@ -817,7 +817,7 @@ static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, const c
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
-    using S = pencil::Shape<N>;                                                                              \
+    using S = pencil::Shape<T, N>;                                                                              \
     const int64_t pencils = (rows + 1) / 2;                                                                  \
     hipLaunchKernelGGL((pencil::x_r2c_kernel<T, TI, N>), dim3((unsigned)((pencils + S::P - 1) / S::P)),     \
                        dim3(256), (pencil_lds<T, N>()), st, in, out, rows, tw);                                \
@@ -835,7 +835,7 @@ static bool pencil_x_c2r(int n, const cpx<T>* in, TO* out, int64_t rows, const c
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
-    using S = pencil::Shape<N>;                                                                              \
+    using S = pencil::Shape<T, N>;                                                                              \
     const int64_t pencils = (rows + 1) / 2;                                                                  \
     hipLaunchKernelGGL((pencil::x_c2r_kernel<T, TO, N>), dim3((unsigned)((pencils + S::P - 1) / S::P)),     \
                        dim3(256), (pencil_lds<T, N>()), st, in, out, rows, tw, scale);                         \
@@ -853,7 +853,7 @@ static bool pencil_strided(int n, int sign, cpx<T>* data, int ni, unsigned no, i
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
-    using S = pencil::Shape<N>;                                                                              \
+    using S = pencil::Shape<T, N>;                                                                              \
     const dim3 grid((unsigned)((ni + S::P - 1) / S::P), no);                                                \
     if (sign < 0)                                                                                            \
       hipLaunchKernelGGL((pencil::strided_kernel<T, N, -1>), grid, dim3(256), (pencil_lds<T, N>()), st, data, \

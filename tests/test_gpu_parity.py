@@ -285,6 +285,26 @@ def test_loadcase_64cubed_200grains_first_increments(product, oracle):
     close(a.field("xi"), b.field("xi"), REL, "xi")
 
 
+@pytest.mark.parametrize("cells", [(32, 32, 32), (32, 24, 64)])
+def test_mixed_precision_larger_pencils(product, oracle, cells):
+    """Mixed form on grids whose FP32 pencils hold 16 points per thread (x, z
+    lengths 32 and 64; y 24 with 12): first three increments of the tension
+    load case against the FP64 oracle, within the bounds of the 16^3 test."""
+    print(f"mixed precision: cells {cells}", flush=True)
+    mat, ori, size = polycrystal(cells, 24, seed=7, ori_seed=8)
+    a = Rve(product, cells, size, mat, ori, precision=1, **EXAMPLE_NUMERICS)
+    b = Rve(oracle, cells, size, mat, ori, **EXAMPLE_NUMERICS)
+    a.init()
+    b.init()
+    load = [tension_x(t=10.0, N=20)]
+    for ra, rb in zip(a.increments(load), b.increments(load)):
+        if ra["increment"] == 3:
+            break
+    assert a.stats().fp32_evaluations > 0
+    close(np.array(a.scalars().P_av), np.array(b.scalars().P_av), 2e-5, "<P>")
+    close(a.field("P"), b.field("P"), 1e-4, "P")
+
+
 def test_mixed_precision_16cubed(product, oracle):
     """Mixed form on the CPU-runnable configuration: FP32 transforms and Gamma
     (FP64 material point, FP64 norms) until the convergence criteria hold,
