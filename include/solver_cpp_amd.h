@@ -275,6 +275,9 @@ typedef struct sc_kernel_times {
   int64_t points[SC_KT_COUNT];    /* summed cells (or wavevectors) processed */
 } sc_kernel_times;
 int sc_rve_profile(sc_rve* rve, int enable);
+/* Profiling of the classes in mask only (bit 1 << SC_KT_*; 0 = off): every
+ * timed class adds two events per launch on the solver's stream. */
+int sc_rve_profile_classes(sc_rve* rve, unsigned mask);
 int sc_rve_get_kernel_times(sc_rve* rve, sc_kernel_times* out);
 
 /* ------------------------------------------------------------------------- */

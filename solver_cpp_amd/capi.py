@@ -166,6 +166,7 @@ _SIGNATURES = {
 # present: measurement hooks and restart checkpoints.
 PROFILE_SIGNATURES = {
     "sc_rve_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "sc_rve_profile_classes": (C.c_int, [C.c_void_p, C.c_uint]),
     "sc_rve_get_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
     "sc_rve_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sc_rve_restore": (C.c_int, [C.c_void_p, C.c_char_p]),

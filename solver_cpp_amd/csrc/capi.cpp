@@ -528,11 +528,13 @@ int sc_rve_reset_stats(sc_rve* R) {
   return 0;
 }
 
-int sc_rve_profile(sc_rve* R, int enable) {
+int sc_rve_profile(sc_rve* R, int enable) { return sc_rve_profile_classes(R, enable ? ~0u : 0u); }
+int sc_rve_profile_classes(sc_rve* R, unsigned mask) {
   return guarded([&] {
     R->r->kt.flush();
-    R->r->kt.on = enable != 0;
-    if (enable) R->r->kt.reset();
+    R->r->kt.on = mask != 0;
+    R->r->kt.mask = mask;
+    if (mask) R->r->kt.reset();
     return 0;
   });
 }

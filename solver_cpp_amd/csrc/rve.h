@@ -37,6 +37,7 @@ enum {
 class KTimer {
  public:
   bool on = false;
+  unsigned mask = ~0u;  // classes timed while on (bit SC_KT_*)
   sc_kernel_times acc{};
   void begin(int cls, hipStream_t st);
   void end(int cls, int64_t points, hipStream_t st);

@@ -225,8 +225,8 @@ hipEvent_t KTimer::get() {
   SCA_HIP(hipEventCreate(&e));
   return e;
 }
-void KTimer::begin(int, hipStream_t st) {
-  if (!on) return;
+void KTimer::begin(int cls, hipStream_t st) {
+  if (!on || !((mask >> cls) & 1u)) return;
   open_a = get();
   SCA_HIP(hipEventRecord(open_a, st));
 }

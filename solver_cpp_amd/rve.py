@@ -209,8 +209,14 @@ class Rve:
         self.lib.check(self.lib.sc_rve_reset_stats(self.h), "reset_stats")
 
     # measurement (HIP library) -------------------------------------------------
-    def profile(self, enable: bool = True):
-        self.lib.check(self.lib.sc_rve_profile(self.h, int(enable)), "profile")
+    def profile(self, enable: bool = True, classes: Optional[Sequence[str]] = None):
+        """Kernel timing with HIP events: every class, or only the named ones
+        (names of KT_CLASSES)."""
+        if enable and classes is not None:
+            mask = sum(1 << KT_CLASSES.index(c) for c in classes)
+            self.lib.check(self.lib.sc_rve_profile_classes(self.h, mask), "profile")
+        else:
+            self.lib.check(self.lib.sc_rve_profile(self.h, int(enable)), "profile")
 
     def kernel_times(self) -> dict:
         k = KernelTimes()
