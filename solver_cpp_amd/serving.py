@@ -126,14 +126,16 @@ def batched_specs(lib, n_rve: int, seed: int = 7, edges=(16, 24, 32, 48), grains
     return out
 
 
-def run_batched(lib, specs, workers: int, budget: int, t: float = 5.0, N: int = 10):
-    """Every RVE through its whole load case on the server; returns totals."""
+def run_batched(lib, specs, workers: int, budget: int, t: float = 5.0, N: int = 10, increments: int = 0):
+    """Every RVE through its load case (the first `increments` of its N
+    increments when given) on the server; returns totals."""
     srv = Server(lib, budget, workers)
     t0 = time.perf_counter()
     reqs = []
+    n_inc = increments if increments > 0 else N
     for spec, q in specs:
         job = srv.add_job(spec, [tension_x(t=t, N=N, rot_q=q)], keep_resident=True)
-        reqs.append([srv.submit(job) for _ in range(N)])
+        reqs.append([srv.submit(job) for _ in range(n_inc)])
         srv.remove_job(job)  # memory is released once its last increment is done
     results = [[srv.wait(r) for r in rs] for rs in reqs]
     wall = time.perf_counter() - t0
@@ -217,17 +219,27 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--max-edge", type=int, default=128)
+    ap.add_argument("--edges", default="16,24,32,48", help="batched: grid edges drawn from (comma separated)")
+    ap.add_argument("--grains", default="10,200", help="batched: grain count range lo,hi")
+    ap.add_argument("--load", default="5,10", help="batched: tension duration t and increments N of the load case")
+    ap.add_argument("--increments", type=int, default=0, help="batched: increments run per RVE (0: all N)")
     args = ap.parse_args()
     lib = Library(PRODUCT_SO, boundary=True)
     prec = 1 if args.precision == "mixed" else 0
     budget = int(args.budget_gb * (1 << 30))
     num = dict(itmin=4, itmax=40)
     if args.mode == "batched":
-        specs = batched_specs(lib, args.rves, precision=prec, **num)
-        r = run_batched(lib, specs, args.workers, budget)
+        edges = tuple(int(e) for e in args.edges.split(","))
+        g_lo, g_hi = (int(g) for g in args.grains.split(","))
+        lt, ln = args.load.split(",")
+        specs = batched_specs(lib, args.rves, edges=edges, grains=(g_lo, g_hi), precision=prec, **num)
+        kw = dict(t=float(lt), N=int(ln), increments=args.increments)
+        r = run_batched(lib, specs, args.workers, budget, **kw)
         print(f"batched: {args.workers} workers done in {r['wall']:.1f} s", file=sys.stderr, flush=True)
-        seq = run_batched(lib, specs, 1, budget)
+        seq = run_batched(lib, specs, 1, budget, **kw)
         print(json.dumps({"mode": "batched", "rves": args.rves, "workers": args.workers, "precision": args.precision,
+                          "edges": list(edges), "grains": [g_lo, g_hi], "load": [float(lt), int(ln)],
+                          "increments": args.increments or int(ln),
                           "mat_point_updates_per_s": r["value"], "wall_s": r["wall"],
                           "one_at_a_time_updates_per_s": seq["value"], "speedup": r["value"] / seq["value"],
                           "max_concurrent": r["max_concurrent"], "peak_gb": r["peak_bytes"] / 2 ** 30,
