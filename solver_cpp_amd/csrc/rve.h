@@ -11,6 +11,9 @@
 
 struct sc_rve;
 
+// grids up to this many cells evaluate their residual as one HIP graph
+constexpr int64_t RESIDUAL_GRAPH_MAX_CELLS = 64 * 64 * 64;
+
 namespace sca {
 
 extern thread_local std::string g_err;
@@ -100,7 +103,9 @@ struct LoadCursor {
 
 class Rve {
  public:
-  explicit Rve(const sc_rve_desc& d);
+  // residual_graphs: grids of at most RESIDUAL_GRAPH_MAX_CELLS cells
+  // evaluate their residual as a HIP graph
+  explicit Rve(const sc_rve_desc& d, bool residual_graphs = true);
   ~Rve();
   Rve(const Rve&) = delete;
   Rve& operator=(const Rve&) = delete;
@@ -111,6 +116,9 @@ class Rve {
   int run_loadcase(const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations, int max_incs);
   int next_increment(const sc_loadstep* steps, int n_steps, LoadCursor& cur, int* iterations, int* evaluations);
   void form_residual(const double* Fdev, double* rdev);
+  void launch_residual(const double* Fdev, double* rdev);
+  bool ensure_residual_graph();  // false: evaluate eagerly this time
+  void release_residual_graph();
   void constitutive_response(const double* Fdev, double dt);
   void fetch_response_scalars(bool rotate);
   void request_response_scalars(bool rotate, bool residual);
@@ -169,6 +177,14 @@ class Rve {
   double* d_scr_partial = nullptr;  // partial sums of the iteration counters
   const double* f_eval = nullptr;   // F of the last response, until sync_response_F
   double* h_scal = nullptr;  // pinned: response scalars in flight
+  // residual evaluation as a HIP graph (grids of at most
+  // RESIDUAL_GRAPH_MAX_CELLS cells): staging of F and r, the graph and the
+  // arguments it was captured with
+  double *g_in = nullptr, *g_out = nullptr;
+  hipGraph_t g_graph = nullptr;
+  hipGraphExec_t g_exec = nullptr;
+  std::vector<char> g_sig, g_seen;
+  int64_t stats_graph_captures = 0, stats_graph_updates = 0;
   bool scal_pending = false, scal_rotate = false, scal_residual = false;
 
   // MechBase state (mech_base.h:75-107), column-major

@@ -231,7 +231,9 @@ class Server {
       res.job = jid;
       try {
         if (admit) {
-          j->rve.reset(new Rve(j->desc));
+          // jobs evaluate eagerly: the per-job streams keep the device busy,
+          // and short jobs would not amortise graph captures
+          j->rve.reset(new Rve(j->desc, false));
           if (j->evicted) {
             j->rve->load(j->snapshot);
             j->snapshot.clear();

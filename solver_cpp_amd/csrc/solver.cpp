@@ -268,7 +268,7 @@ double* Rve::dalloc(size_t n) {
   return p;
 }
 
-Rve::Rve(const sc_rve_desc& d) {
+Rve::Rve(const sc_rve_desc& d, bool residual_graphs) {
   desc = d;
   for (int p = 0; p < 3; ++p) {
     cells[p] = d.cells[p];
@@ -337,6 +337,10 @@ Rve::Rve(const sc_rve_desc& d) {
   dPdF = dalloc(81 * N);
   partial = dalloc((size_t)std::max<int64_t>(red_blocks() * MAX_RED_VEC, 9 * mp_partial_blocks(N)) +
                    gamma_blocks(K));
+  if (residual_graphs && N <= RESIDUAL_GRAPH_MAX_CELLS) {  // staging of the captured residual evaluation
+    g_in = dalloc(9 * N);
+    g_out = dalloc(9 * N);
+  }
   small = dalloc(512);
   d_u = dalloc(3 * N);
   d_xp = dalloc(3 * N);
@@ -415,6 +419,7 @@ Rve::Rve(const sc_rve_desc& d) {
 }
 
 Rve::~Rve() {
+  release_residual_graph();
   delete ngm;
   if (h_scal) (void)hipHostFree(h_scal);
   if (spec32) (void)hipFree(spec32);
@@ -569,10 +574,28 @@ GammaHost Rve::gamma_host(bool do_gamma) const {
 }
 
 // MechSolverBasic::form_residual, mech_solver_basic.cpp:286-339
-void Rve::form_residual(const double* Fdev, double* rdev) {
-  if (nfuncs == 0 && snes_iter == 0) total_iter = iter_base - 1;
-  if (total_iter - iter_base <= snes_iter) total_iter += 1;
-  constitutive_response(Fdev, delta_t);
+// The device work of one residual evaluation on st, without host
+// synchronisation: material-point update at F, P sums, the aim of the
+// convolution, forward transform, Gamma convolution with the divergence,
+// inverse transform into r, and the asynchronous copy of the response
+// scalars.  Its launches depend on F, r, delta_t, the compliance S, the load
+// rotation, C_ref and the precision phase only, so it can be captured.
+void Rve::launch_residual(const double* Fdev, double* rdev) {
+  MpState s = mp;
+  s.F = const_cast<double*>(Fdev);  // read only
+  kt.begin(SC_KT_RESPONSE, st);
+  launch_mp_response(s, mc, delta_t, d_ill, sub, st);
+  kt.end(SC_KT_RESPONSE, N, st);
+  kt.begin(SC_KT_REDUCE, st);
+  launch_pav_partial(mp.P, N, partial, st);
+  launch_final_reduce(partial, 9, mp_partial_blocks(N), small + SM_SUMS, st);
+  kt.end(SC_KT_REDUCE, N, st);
+  if (kt.on) {  // iteration counts of this evaluation
+    const int ops[2] = {2, 2};
+    const int* it[2] = {mp.iters, mp.iters + N};
+    launch_partial_reduce<int>(ops, N, 2, it, nullptr, d_scr_partial, st);
+    launch_final_reduce(d_scr_partial, 2, red_blocks(), small + SM_ITERS, st);
+  }
   // DC of the convolution: rotated S : P_av, computed on the device
   double R[9], Rrm[9];
   qu2om_host(rot_q, R);
@@ -590,8 +613,6 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
     kt.begin(SC_KT_FFT_BWD, st);
     fft32.backward(spec32, rdev, 9, 1.0, st);
     kt.end(SC_KT_FFT_BWD, N, st);
-    evals32++;
-    stats.fp32_evaluations++;
   } else {
     kt.begin(SC_KT_FFT_FWD, st);
     fft.forward(mp.P, spec, 9, st);
@@ -603,11 +624,105 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
     kt.begin(SC_KT_FFT_BWD, st);
     fft.backward(spec, rdev, 9, 1.0, st);
     kt.end(SC_KT_FFT_BWD, N, st);
-    evals64++;
   }
+  static_assert(SM_ILL == SM_SUMS + 9 && SM_DIV == SM_ILL + 1 && SM_ITERS == SM_DIV + 1,
+                "response scalars must be adjacent");
+  SCA_HIP(hipMemcpyAsync(h_scal, small + SM_SUMS, (kt.on ? 13 : 11) * sizeof(double), hipMemcpyDeviceToHost, st));
+}
+
+// Graph of launch_residual(g_in, g_out) for the current arguments.  A
+// signature seen for the first time is evaluated eagerly (returns false): a
+// capture pays off only when the arguments repeat.  A new capture updates the
+// executable graph in place when only kernel arguments changed (the launch
+// sequence is the same), instead of instantiating it again.
+bool Rve::ensure_residual_graph() {
+  struct Sig {
+    double dt, S[81], rot_q[4], C_ref[81];
+    int use32;
+  } sig{};
+  sig.dt = delta_t;
+  std::memcpy(sig.S, S, sizeof sig.S);
+  std::memcpy(sig.rot_q, rot_q, sizeof sig.rot_q);
+  std::memcpy(sig.C_ref, C_ref, sizeof sig.C_ref);
+  sig.use32 = use32 ? 1 : 0;
+  const char* sb = reinterpret_cast<const char*>(&sig);
+  auto same = [&](const std::vector<char>& v) { return v.size() == sizeof sig && std::memcmp(v.data(), sb, sizeof sig) == 0; };
+  if (g_exec && same(g_sig)) return true;
+  if (!same(g_seen)) {
+    g_seen.assign(sb, sb + sizeof sig);
+    return false;
+  }
+  SCA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    launch_residual(g_in, g_out);
+  } catch (...) {
+    hipGraph_t broken = nullptr;
+    (void)hipStreamEndCapture(st, &broken);
+    if (broken) (void)hipGraphDestroy(broken);
+    throw;
+  }
+  hipGraph_t graph = nullptr;
+  SCA_HIP(hipStreamEndCapture(st, &graph));
+  if (g_exec) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(g_exec, graph, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      if (g_graph) (void)hipGraphDestroy(g_graph);
+      g_graph = graph;
+      g_sig.assign(sb, sb + sizeof sig);
+      stats_graph_updates++;
+      return true;
+    }
+    (void)hipGetLastError();  // a refused update leaves no error behind
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    SCA_HIP(e);
+  }
+  release_residual_graph();
+  g_graph = graph;
+  g_exec = exec;
+  g_sig.assign(sb, sb + sizeof sig);
+  stats_graph_captures++;
+  return true;
+}
+
+void Rve::release_residual_graph() {
+  if (g_exec) (void)hipGraphExecDestroy(g_exec);
+  if (g_graph) (void)hipGraphDestroy(g_graph);
+  g_exec = nullptr;
+  g_graph = nullptr;
+  g_sig.clear();
+}
+
+// MechSolverBasic::formResidual (mech_solver_basic.cpp:286-339).  Small grids
+// are bound by launch latency: there the evaluation is one graph launch
+// between copies of F in and r out (not while kernels are being timed).
+void Rve::form_residual(const double* Fdev, double* rdev) {
+  if (nfuncs == 0 && snes_iter == 0) total_iter = iter_base - 1;
+  if (total_iter - iter_base <= snes_iter) total_iter += 1;
+  if (g_in && !kt.on && ensure_residual_graph()) {
+    SCA_HIP(hipMemcpyAsync(g_in, Fdev, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    SCA_HIP(hipGraphLaunch(g_exec, st));
+    SCA_HIP(hipMemcpyAsync(rdev, g_out, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+  } else {
+    launch_residual(Fdev, rdev);
+  }
+  f_eval = Fdev;
+  last_dt = delta_t;
   // P_av, err_div, F_aim and err_BC of this evaluation are completed at the
   // next synchronisation (complete_response_scalars)
-  request_response_scalars(true, true);
+  scal_pending = true;
+  scal_rotate = true;
+  scal_residual = true;
+  if (use32) {
+    evals32++;
+    stats.fp32_evaluations++;
+  } else {
+    evals64++;
+  }
   nfuncs++;
   solve_evals++;
   stats.residual_evaluations++;

@@ -305,6 +305,31 @@ def test_mixed_precision_larger_pencils(product, oracle, cells):
     close(a.field("P"), b.field("P"), 1e-4, "P")
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_residual_graph_replays_match_eager(product, precision):
+    """Small grids evaluate the residual as a captured HIP graph, replayed
+    many times per increment between the solver's own eager launches and
+    host synchronisations, and captured again when its arguments change
+    (time step, compliance, precision phase).  Kernel timing forces the
+    eager launches: both runs must agree bitwise in every increment."""
+    print(f"graph vs eager: precision {precision}", flush=True)
+    cells = (16, 12, 8)
+    mat, ori, size = polycrystal(cells, 9, seed=21, ori_seed=22)
+    runs = []
+    for timed in (False, True):
+        r = Rve(product, cells, size, mat, ori, precision=precision, **EXAMPLE_NUMERICS)
+        r.init()
+        r.profile(timed)
+        trace = [(d["iterations"], d["evaluations"], r.scalars().P_av[:])
+                 for d in r.increments([tension_x(t=4.0, N=8)])]
+        runs.append((trace, r.field("F"), r.field("P"), r.field("xi")))
+    (ta, Fa, Pa, xa), (tb, Fb, Pb, xb) = runs
+    assert ta == tb
+    np.testing.assert_array_equal(Fa, Fb)
+    np.testing.assert_array_equal(Pa, Pb)
+    np.testing.assert_array_equal(xa, xb)
+
+
 def test_mixed_precision_16cubed(product, oracle):
     """Mixed form on the CPU-runnable configuration: FP32 transforms and Gamma
     (FP64 material point, FP64 norms) until the convergence criteria hold,
