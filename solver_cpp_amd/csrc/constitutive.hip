@@ -448,11 +448,15 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
                                                 const PointIO& io, const MatConst& mc) {
   const int64_t N = io.N;
   const Lane l = io.l;
-  // A is read from LDS where it is used (volatile: not hoisted into the
-  // registers the Newton system needs)
+  // A is read from LDS where it is used, not hoisted into the registers the
+  // Newton system needs: the lane index passes through an empty asm, so every
+  // call's nine ds_reads are new loads (issued together, one wait), not the
+  // serialised one-wait-each accesses a volatile read would be
   auto loadA = [&](double* A) {
+    int tl = t;
+    asm volatile("" : "+v"(tl));
 #pragma unroll
-    for (int k = 0; k < 9; ++k) A[k] = ((const volatile double*)hA[k])[t];
+    for (int k = 0; k < 9; ++k) A[k] = hA[k][tl];
   };
   double Lpg_old[9], dLp[9];
   double res_old_n2 = 0.0;  // squared norms: the tests compare non-negative values
