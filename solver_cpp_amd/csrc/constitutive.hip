@@ -753,11 +753,15 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
         }
         for (int it = 1; it <= mc.N_iter_state_max; ++it) {
           fpi++;
+          // history shift of the first iteration (last2 = 0); later
+          // iterations shift it at the end of the previous one, where last1
+          // is already in registers
+          if (it == 1) {
 #pragma unroll
-          for (int k = 0; k < NSTATE; ++k) {
-            const double l2 = (it > 1) ? h_last1[k][t] : 0.0;
-            h_d12[k][t] = dot[k] - l2;  // last1 - last2 after the shift
-            h_last1[k][t] = dot[k];
+            for (int k = 0; k < NSTATE; ++k) {
+              h_d12[k][t] = dot[k];  // last1 - last2 after the shift
+              h_last1[k][t] = dot[k];
+            }
           }
 #pragma unroll
           for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
@@ -805,6 +809,8 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
             // the next iteration (phase_mechanical.f90:653-655)
             const double dk = dot[k] * zeta + l1 * (1.0 - zeta);
             dot[k] = dk;
+            h_d12[k][t] = dk - l1;  // the next iteration's shift
+            h_last1[k][t] = dk;
             const double cur = (k < NSLIP) ? xi[k] : h_gam[k - NSLIP][t];
             const double r = cur - s0[k] - dk * sdt;
             const double nw = cur - r;
