@@ -746,10 +746,14 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
 #pragma unroll
       for (int k = 0; k < NSTATE; ++k) nan = nan || isnan(dot[k]);
       if (!nan) {
+        // start-of-step state, loaded once for the whole fixed point
+        double s0[NSTATE];
+#pragma unroll
+        for (int k = 0; k < NSTATE; ++k) s0[k] = st0(k);
 #pragma unroll
         for (int k = 0; k < NSLIP; ++k) {
-          xi[k] = st0(k) + dot[k] * sdt;
-          h_gam[k][t] = st0(NSLIP + k) + dot[NSLIP + k] * sdt;
+          xi[k] = s0[k] + dot[k] * sdt;
+          h_gam[k][t] = s0[NSLIP + k] + dot[NSLIP + k] * sdt;
         }
         for (int it = 1; it <= mc.N_iter_state_max; ++it) {
           fpi++;
@@ -770,11 +774,6 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
           refused = rc == 2;
           broken = rc != 0;
           if (broken) break;
-          // start-of-step state for the update below, loaded ahead of the
-          // slip-rate evaluation so that its latency is hidden
-          double s0[NSTATE];
-#pragma unroll
-          for (int k = 0; k < NSTATE; ++k) s0[k] = st0(k);
           {
             double stt[NSTATE];
 #pragma unroll
