@@ -374,11 +374,18 @@ __device__ __forceinline__ void store33(double* base, int64_t N, Lane l, const d
 #pragma unroll
   for (int q = 0; q < 9; ++q) st(base + q * N, l, m[rm(q)]);
 }
+// load from a plane chosen per lane (the base may differ between lanes): a
+// 64-bit lane address and one unconditional load, so that the loads of all
+// lanes and planes issue together instead of in divergent branches
+__device__ __forceinline__ double ldv(const double* plane, Lane l) {
+  return *reinterpret_cast<const gdouble*>(reinterpret_cast<uint64_t>(plane) + l.off);
+}
 // per-lane choice of the plane set: sel ? a : b
 __device__ __forceinline__ void load33_sel(bool sel, const double* a, const double* b, int64_t N, Lane l,
                                            double* m) {
+  const double* base = sel ? a : b;
 #pragma unroll
-  for (int q = 0; q < 9; ++q) m[rm(q)] = sel ? ld(a + q * N, l) : ld(b + q * N, l);
+  for (int q = 0; q < 9; ++q) m[rm(q)] = ldv(base + q * N, l);
 }
 
 // J = I + dt sum_s c_s vec(P_s) vec(G^T K_s)^T with G = Fe^T A, restricted to
@@ -658,7 +665,8 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
   bool in_sub = false;
   int newton = 0, fpi = 0;
   auto st0 = [&](int k) -> double {
-    return in_sub ? ld(sub + (SUB_ST0 + k) * N, l) : (k < NSLIP ? ld(s.xi0 + (k) * N, l) : ld(s.gamma0 + (k - NSLIP) * N, l));
+    if (k < NSLIP) return ldv((in_sub ? sub + SUB_ST0 * N : s.xi0) + k * N, l);
+    return ldv((in_sub ? sub + (SUB_ST0 + NSLIP) * N : s.gamma0) + (k - NSLIP) * N, l);
   };
 
   // F of the (sub)step: F0l + step (F - F0), phase_mechanical.f90:1093-1096
@@ -679,16 +687,26 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
   bool converged = false, todo = true;
   bool refused = false;  // first pass only: the Newton system needs row exchanges
   while (todo) {
+    double fp0v[9];  // Fp0 of the (sub)step about to be integrated (planes in order)
     if (converged) {
       stepFrac = stepFrac + step;
       step = fmin(1.0 - stepFrac, mc.r_increase * step);
       todo = step > 0.0;
       if (todo) {
+        // all loads before the stores: the compiler cannot tell the planes
+        // apart, and a load after a store would wait for the one before it
+        double fl[9], fp[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-          st(sub + (SUB_F0 + k) * N, l, ld(sub + (SUB_FLOC + k) * N, l));
+          fl[k] = ld(sub + (SUB_FLOC + k) * N, l);
+          fp[k] = ld(s.Fp + (k) * N, l);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          st(sub + (SUB_F0 + k) * N, l, fl[k]);
           st(sub + (SUB_LP0 + k) * N, l, Lp[rm(k)]);
-          st(sub + (SUB_FP0 + k) * N, l, ld(s.Fp + (k) * N, l));
+          st(sub + (SUB_FP0 + k) * N, l, fp[k]);
+          fp0v[k] = fp[k];
         }
 #pragma unroll
         for (int k = 0; k < NSLIP; ++k) {
@@ -701,13 +719,15 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
       step = mc.r_cutback * step;
       load33(s.S0, N, l, S);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) st(s.Fp + (k) * N, l, in_sub ? ld(sub + (SUB_FP0 + k) * N, l) : ld(s.Fp0 + (k) * N, l));
+      for (int k = 0; k < 9; ++k) fp0v[k] = ldv((in_sub ? sub + SUB_FP0 * N : s.Fp0) + k * N, l);
       if (step < 1.0) load33_sel(in_sub, sub + SUB_LP0 * N, s.Lp0, N, l, Lp);
 #pragma unroll
       for (int k = 0; k < NSLIP; ++k) {
         xi[k] = st0(k);
         h_gam[k][t] = st0(NSLIP + k);
       }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) st(s.Fp + (k) * N, l, fp0v[k]);  // after the loads, as above
       todo = step > mc.r_cutback_min;
     }
     if (todo) {
@@ -718,7 +738,8 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
         double Fl[9], Fp0[9], iF0[9], A[9];
         load_Floc(step, Fl);
         if (step < 1.0) store33(sub + SUB_FLOC * N, N, l, Fl);  // start of the next substep
-        load33_sel(in_sub, sub + SUB_FP0 * N, s.Fp0, N, l, Fp0);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fp0[rm(k)] = fp0v[k];
         fp0_singular = !inv33(Fp0, iF0, nullptr);
         mm33(Fl, iF0, A);
 #pragma unroll
