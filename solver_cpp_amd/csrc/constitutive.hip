@@ -652,13 +652,9 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
   double* sub = a.sub;
   double Lp[9], S[9];
   load33(s.Lp, N, l, Lp);
-  load33(s.S0, N, l, S);  // phase_restore: S = S0, Fp = Fp0, state = state0
+  // S, xi and the accumulated shear are set by the restore that opens the
+  // first step (phase_restore: S = S0, Fp = Fp0, state = state0)
   double xi[NSLIP];
-#pragma unroll
-  for (int k = 0; k < NSLIP; ++k) {
-    xi[k] = ld(s.xi0 + (k) * N, l);
-    h_gam[k][t] = ld(s.gamma0 + (k) * N, l);
-  }
   // substep start values live in memory: the increment's converged values
   // (F0, Lp0, Fp0, state0) until a partial substep succeeded, then the
   // substep scratch planes
@@ -669,11 +665,18 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
     return ldv((in_sub ? sub + (SUB_ST0 + NSLIP) * N : s.gamma0) + (k - NSLIP) * N, l);
   };
 
+  // F and F0 of the increment (not written here), loaded once with Lp
+  double Fi[9], F0i[9];
+  load33(s.F, N, l, Fi);
+  load33(s.F0, N, l, F0i);
   // F of the (sub)step: F0l + step (F - F0), phase_mechanical.f90:1093-1096
   auto load_Floc = [&](double st_, double* Fl) {
     double F[9], F0l[9], F0p[9];
-    load33(s.F, N, l, F);
-    load33(s.F0, N, l, F0p);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      F[k] = Fi[k];
+      F0p[k] = F0i[k];
+    }
     if (in_sub) {
       load33(sub + SUB_F0 * N, N, l, F0l);
     } else {
