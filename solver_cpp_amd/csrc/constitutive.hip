@@ -770,10 +770,12 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
 #pragma unroll
       for (int k = 0; k < NSTATE; ++k) nan = nan || isnan(dot[k]);
       if (!nan) {
-        // start-of-step state, loaded once for the whole fixed point
+        // start-of-step state for the whole fixed point: xi and the shear
+        // column hold it here (the restore above loaded them from the start
+        // planes, or a new substep stored its start planes from them)
         double s0[NSTATE];
 #pragma unroll
-        for (int k = 0; k < NSTATE; ++k) s0[k] = st0(k);
+        for (int k = 0; k < NSTATE; ++k) s0[k] = k < NSLIP ? xi[k] : h_gam[k - NSLIP][t];
 #pragma unroll
         for (int k = 0; k < NSLIP; ++k) {
           xi[k] = s0[k] + dot[k] * sdt;
