@@ -956,6 +956,8 @@ __device__ __forceinline__ double Cijkl(const TangentArgs& a, int i, int j, int 
 }
 }  // namespace
 
+// NE4: the slip-rate exponent n_sl as a compile-time quarter (-1: runtime)
+template <int NE4>
 __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
   const MatConst& mc = a.mc;
   const MpState& s = a.s;
@@ -983,7 +985,7 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
   double dg[NSLIP], cs[NSLIP], rxi[NSLIP];
 #pragma unroll
   for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
-  kinetics<-1>(S, rxi, dg, cs, mc);
+  kinetics<NE4>(S, rxi, dg, cs, mc);
 
   // rhs(p,o,:,:) = Fe C(p,o,:,:) invFp^T  -> R[(p,o)][(k,l)]
   double R[81];
@@ -1097,40 +1099,9 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
       }
   }
   // R now holds dSdF[(i,j)][(p,o)].
-  // V_s(p,o) = P_s : dSdF(:,:,p,o); dFpinvdF(:,:,p,o) = -dt sum_s c_s V_s(p,o) (invFp0 P_s)
-  double D[81];  // dFpinvdF[(k,l)][(p,o)]
-#pragma unroll
-  for (int k = 0; k < 81; ++k) D[k] = 0.0;
-#pragma unroll
-  for (int sl = 0; sl < NSLIP; ++sl) {
-    double V[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      double t = 0.0;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          if (psg(sl, i, j) != 0) t += (mc.v * (double)psg(sl, i, j)) * R[9 * (3 * i + j) + q];
-      V[q] = t;
-    }
-    double Q[9];  // invFp0 P_s
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        double t = 0.0;
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-          if (psg(sl, m, l) != 0) t += invFp0[3 * k + m] * (mc.v * (double)psg(sl, m, l));
-        Q[3 * k + l] = t;
-      }
-    const double f = -dt * cs[sl];
-#pragma unroll
-    for (int kl = 0; kl < 9; ++kl)
-#pragma unroll
-      for (int q = 0; q < 9; ++q) D[9 * kl + q] += f * Q[kl] * V[q];
-  }
+  // dFpinvdF(:,:,p,o) = -dt sum_s c_s V_s(p,o) (invFp0 P_s) with
+  // V_s(p,o) = P_s : dSdF(:,:,p,o), formed column by column below (one
+  // column of dSdF at a time, so that no third 81-entry array is live)
   // assemble dPdF
   double SiT[9], iFt[9], FiFp[9], u3[9];
 #pragma unroll
@@ -1154,8 +1125,31 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
       double dF[9], dS[9];
 #pragma unroll
       for (int kl = 0; kl < 9; ++kl) {
-        dF[kl] = D[9 * kl + q];
+        dF[kl] = 0.0;
         dS[kl] = R[9 * kl + q];
+      }
+#pragma unroll
+      for (int sl = 0; sl < NSLIP; ++sl) {
+        double V = 0.0;  // V_s(p,o)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            if (psg(sl, i, j) != 0) V += (mc.v * (double)psg(sl, i, j)) * dS[3 * i + j];
+        double Q[9];  // invFp0 P_s
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) {
+            double t = 0.0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+              if (psg(sl, m, l) != 0) t += invFp0[3 * k + m] * (mc.v * (double)psg(sl, m, l));
+            Q[3 * k + l] = t;
+          }
+        const double f = -dt * cs[sl];
+#pragma unroll
+        for (int kl = 0; kl < 9; ++kl) dF[kl] += f * Q[kl] * V;
       }
       double a1[9], a2[9], a3[9], t1[9], t2[9], dFt[9];
       mm33(F, dF, t1);
@@ -1289,7 +1283,11 @@ void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* 
           a.C[27 * i + 9 * j + 3 * k + l] = v;
         }
   const int blocks = (int)((c_end - c_begin + 63) / 64);
-  hipLaunchKernelGGL(mp_tangent_kernel, dim3(blocks), dim3(64), 0, st, a);
+  // the example material's slip-rate exponent (n_sl = 20) is compiled in
+  // (the same products as the runtime binary powering, without the pow()
+  // call path whose register saves spill)
+  if (mc.n_e4 == 80) hipLaunchKernelGGL(mp_tangent_kernel<80>, dim3(blocks), dim3(64), 0, st, a);
+  else hipLaunchKernelGGL(mp_tangent_kernel<-1>, dim3(blocks), dim3(64), 0, st, a);
   SCA_HIP(hipGetLastError());
 }
 
