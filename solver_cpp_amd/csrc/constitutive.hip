@@ -1283,9 +1283,10 @@ void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* 
           a.C[27 * i + 9 * j + 3 * k + l] = v;
         }
   const int blocks = (int)((c_end - c_begin + 63) / 64);
-  // the example material's slip-rate exponent (n_sl = 20) is compiled in
-  // (the same products as the runtime binary powering, without the pow()
-  // call path whose register saves spill)
+  // the example material's slip-rate exponent (n_sl = 20) is compiled in:
+  // the slip rates in the response kernel's form (x^(n-1) from one powering
+  // chain, c_s without a division; equal to the runtime form to rounding),
+  // and no pow() call path, whose register saves spill
   if (mc.n_e4 == 80) hipLaunchKernelGGL(mp_tangent_kernel<80>, dim3(blocks), dim3(64), 0, st, a);
   else hipLaunchKernelGGL(mp_tangent_kernel<-1>, dim3(blocks), dim3(64), 0, st, a);
   SCA_HIP(hipGetLastError());
