@@ -124,9 +124,11 @@ def test_response_reductions(product, oracle):
 
 
 # ---- material point ------------------------------------------------------------------
-def _mp_batch(lib, F, Fp0, dt, Lp=None, xi0=None):
+def _mp_batch(lib, F, Fp0, dt, Lp=None, xi0=None, material=None):
     n = F.shape[0]
     m, p, _ = lib.defaults()
+    if material:
+        material(m)
     I9 = np.tile(np.eye(3).reshape(-1), (n, 1))
     Z9 = np.zeros((n, 9))
     xi0 = np.full((n, 12), m.xi_0_sl) if xi0 is None else xi0
@@ -173,6 +175,29 @@ def test_mp_update(product, oracle, strain):
     b2 = _mp_batch(oracle, F, Fp0, 0.5, Lp=b["Lp"])
     np.testing.assert_array_equal(a2["iterations"], b2["iterations"])
     close(a2["P"], b2["P"], REL, "P (warm start)")
+
+
+@pytest.mark.parametrize("n_sl,a_sl", [(10.0, 2.0), (17.3, 1.7)])
+def test_mp_update_other_exponents(product, oracle, n_sl, a_sl):
+    """Materials other than the example's n_sl = 20, a_sl = 2.25 run the
+    runtime-exponent kernels: quarter exponents by binary powering, others
+    by pow(); the oracle takes pow() throughout.  Same broken flags, the
+    same iteration counts for at least 99 % of the points (a rounding-level
+    difference of the powers may move a stopping decision) and every output
+    of those within 1e-10, the consistent tangent included."""
+    F, Fp0 = _random_batch(oracle, 1e-3)
+
+    def material(m):
+        m.n_sl, m.a_sl = n_sl, a_sl
+
+    a = _mp_batch(product, F, Fp0, 0.5, material=material)
+    b = _mp_batch(oracle, F, Fp0, 0.5, material=material)
+    np.testing.assert_array_equal(a["broken"], b["broken"])
+    assert not b["broken"].any()
+    same = (a["iterations"] == b["iterations"]).all(axis=1)
+    assert same.mean() >= 0.99, same.mean()
+    for k in ["P", "Fp", "Lp", "S", "Fe", "xi", "gamma", "dPdF"]:
+        close(a[k][same], b[k][same], REL, k)
 
 
 def test_mp_update_substepping(product, oracle):
