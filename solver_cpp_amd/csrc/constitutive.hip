@@ -914,12 +914,23 @@ __global__ void __launch_bounds__(256) pav_partial_kernel(const double* P, int64
   double acc[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  if ((int64_t)(blockIdx.x + 1) * PAV_CELLS <= N) {
+    // interior block: no per-element test, so every load can be issued
+    // ahead of the sums (same summation order)
 #pragma unroll
-  for (int u = 0; u < PAV_CELLS / 256; ++u) {
-    const int64_t c = c0 + (int64_t)u * 256;
-    if (c < N) {
+    for (int u = 0; u < PAV_CELLS / 256; ++u) {
+      const int64_t c = c0 + (int64_t)u * 256;
 #pragma unroll
       for (int q = 0; q < 9; ++q) acc[q] += P[q * N + c];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < PAV_CELLS / 256; ++u) {
+      const int64_t c = c0 + (int64_t)u * 256;
+      if (c < N) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] += P[q * N + c];
+      }
     }
   }
 #pragma unroll
