@@ -502,6 +502,8 @@ __global__ void __launch_bounds__(256) partial_reduce_kernel(RedArgs r, double* 
   const T* a = reinterpret_cast<const T*>(r.a[v]);
   const T* b = reinterpret_cast<const T*>(r.b[v]);
   const int op = r.op[v];
+  // a squared norm (b == a) reads its vector once
+  const bool own = (op == 2) || (a == b);
   double s = 0.0;
   constexpr int64_t STRIDE = (int64_t)RED_BLOCKS * 256;
   int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -510,10 +512,9 @@ __global__ void __launch_bounds__(256) partial_reduce_kernel(RedArgs r, double* 
   for (; t + 3 * STRIDE < r.n; t += 4 * STRIDE) {
     double x[4], y[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      x[u] = (double)a[t + u * STRIDE];
-      y[u] = (op == 2) ? 0.0 : (double)b[t + u * STRIDE];
-    }
+    for (int u = 0; u < 4; ++u) x[u] = (double)a[t + u * STRIDE];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) y[u] = own ? x[u] : (double)b[t + u * STRIDE];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (op == 0) s += x[u] * y[u];
@@ -525,9 +526,10 @@ __global__ void __launch_bounds__(256) partial_reduce_kernel(RedArgs r, double* 
   }
   for (; t < r.n; t += STRIDE) {
     const double x = (double)a[t];
-    if (op == 0) s += x * (double)b[t];
+    const double y = own ? x : (double)b[t];
+    if (op == 0) s += x * y;
     else if (op == 1) {
-      const double d = x - (double)b[t];
+      const double d = x - y;
       s += d * d;
     } else s += x;
   }
