@@ -7,17 +7,25 @@ coordinate update) on a 128^3 periodic polycrystal of 1000 Voronoi grains
 stresses zero, 60 increments to 3 % strain).  All fields stay resident in HBM;
 the timed region covers whole increments.
 
-The metric is material-point updates per second: cells x residual
-evaluations over the timed increments / wall time, plus the p50 / p99
-increment latency.  One JSON line is printed on stdout.
+The metric is material-point updates per second: cells x material-point
+updates (constitutive evaluations) over the timed increments / wall time,
+plus the p50 / p99 increment latency.  The plain FP64 form runs the same
+increments for the comparison; the equal-work comparison is the increment
+time.  A second part runs the batched workload (256 RVEs of uneven grids
+through their load cases) through the batched job server, in both
+precisions, against the same RVEs run one at a time in FP64.  One JSON line
+is printed on stdout.
 
     python bench.py [--steps K] [--warmup W] [--precision fp64|mixed]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -37,23 +45,22 @@ METRIC = "mat-point updates/s @128^3 vs plain FP64 GPU; p50/p99 increment latenc
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic work per unit, per kernel class:
-#   response: bytes of the state read and written per cell and evaluation
-#             (F F0 Fp0 Lp S0 xi0 gamma0 in, Lp S P Fp Fe xi gamma out, 2 counters)
-#             and FP64 flops per Newton iteration / state iteration / cell
-#   fft:      9 real planes in, 9 half-spectrum planes out (per cell)
-#   gamma:    9 complex values in and out per wavevector
-RESPONSE_BYTES_PER_CELL = (9 * 5 + 12 * 2 + 9 * 5 + 12 * 2) * 8 + 8
-RESPONSE_FLOPS_NEWTON = 2800  # 8x8 traceless solve: Fe, Hooke, 12 slip rates, residual, assembly 1610, LU 470
+# Algorithmic work of the material-point update (own count of this
+# formulation; the kernel counts its Newton and state iterations): FP64
+# flops per Newton iteration (Fe, Hooke, 12 slip rates, residual, traceless
+# 8x8 Jacobian assembly 1610, LU 470), per state iteration and per cell, and
+# bytes of state read and written per cell and evaluation (F F0 Fp0 Lp S0
+# xi0 gamma0 in, Lp S P Fp Fe xi gamma out, 2 counters).  The executed FP64
+# instructions counted by rocprofv3 on the same command are recorded next to
+# these in profiles/.
+RESPONSE_FLOPS_NEWTON = 2800
 RESPONSE_FLOPS_STATE = 850
 RESPONSE_FLOPS_CELL = 300
-FFT_BYTES_PER_CELL = 9 * 8 + 9 * 16 / 2
-GAMMA_BYTES_PER_WAVEVECTOR = 2 * 9 * 16
+RESPONSE_BYTES_PER_CELL = (9 * 5 + 12 * 2 + 9 * 5 + 12 * 2) * 8 + 8
 
-
-# HBM bytes per launch of a kernel on this workload, measured by rocprofv3
-# FETCH_SIZE / WRITE_SIZE passes over this script and scaled by the same
-# counters on a copy kernel of known size; recorded in profiles/
+# HBM bytes per launch of the response kernel on this workload, measured by
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this script and scaled by the
+# same counters on a copy kernel of known size; recorded in profiles/
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_128cubed_1000grains.md")
 
 
@@ -74,15 +81,18 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(args, loads, mat, ori, size, precision):
+def run_gpu(args, loads, mat, ori, size, precision, **num):
     lib = Library(PRODUCT_SO)
     if lib.backend != "hip-gfx950":
         raise RuntimeError(f"unexpected backend {lib.backend}")
     cells = (args.cells,) * 3
-    rve = Rve(lib, cells, size, mat, ori, precision=1 if precision == "mixed" else 0, **EXAMPLE_NUMERICS)
+    numerics = dict(EXAMPLE_NUMERICS)
+    numerics.update(num)
+    rve = Rve(lib, cells, size, mat, ori, precision=1 if precision == "mixed" else 0, **numerics)
     t0 = time.perf_counter()
     rve.init()
-    log(f"[bench] {precision}: init {cells} with {args.grains} grains: {time.perf_counter() - t0:.2f} s")
+    log(f"[bench] {precision}{' tight' if num else ''}: init {cells} with {args.grains} grains: "
+        f"{time.perf_counter() - t0:.2f} s")
     it = rve.increments(loads)
     for _ in range(args.warmup):
         r = next(it)
@@ -106,57 +116,142 @@ def run_gpu(args, loads, mat, ori, size, precision):
     rve.profile(False)
     N = int(np.prod(cells))
     out = dict(N=N, wall=wall, lat=lat, evals=evals, kt=kt, stats=st, P_av=np.array(rve.scalars().P_av),
-               P=rve.field("P"))
+               P=rve.field("P"), updates=int(st.material_point_updates))
     rve.close()
     return out
 
 
 def roofline(res):
+    """The response kernel (dominant): algorithmic FP64 flops per launch over
+    its average HIP-event time on the solver's stream."""
     kt, st = res["kt"], res["stats"]
-    dom = max(kt, key=lambda k: kt[k]["ms"])
-    d = kt[dom]
-    avg_s = d["ms"] / max(d["launches"], 1) * 1e-3
-    per_launch_units = d["points"] / max(d["launches"], 1)
-    out = {"kernel": dom, "launches": d["launches"], "avg_ms": d["ms"] / max(d["launches"], 1)}
-    if dom == "response":
-        flops = (st.newton_iterations * RESPONSE_FLOPS_NEWTON + st.state_iterations * RESPONSE_FLOPS_STATE +
-                 d["points"] * RESPONSE_FLOPS_CELL) / max(d["launches"], 1)
-        ach = flops / avg_s / 1e12
-        out.update(bound="mfma", achieved=ach, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=ach / FP64_PEAK_TFLOPS,
-                   note="FP64 vector work (no MFMA); peak is MI355X's FP64 78.6 TFLOP/s",
-                   hbm_gbs=RESPONSE_BYTES_PER_CELL * per_launch_units / avg_s / 1e9)
-    else:
-        per_unit = {"fft_fwd": FFT_BYTES_PER_CELL, "fft_bwd": FFT_BYTES_PER_CELL,
-                    "gamma": GAMMA_BYTES_PER_WAVEVECTOR}.get(dom, 16.0)
-        ach = per_unit * per_launch_units / avg_s / 1e9
-        out.update(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS)
-    out["traffic"] = recorded_traffic(dom)
-    return out
+    d = kt["response"]
+    launches = max(d["launches"], 1)
+    avg_s = d["ms"] / launches * 1e-3
+    cells = d["points"] / launches
+    flops = (st.newton_iterations * RESPONSE_FLOPS_NEWTON + st.state_iterations * RESPONSE_FLOPS_STATE +
+             d["points"] * RESPONSE_FLOPS_CELL) / launches
+    ach = flops / avg_s / 1e12
+    return {"kernel": "response", "launches": d["launches"], "avg_ms": d["ms"] / launches,
+            "bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / FP64_PEAK_TFLOPS,
+            "note": "FP64 vector ALU work (no dense contraction, no MFMA): peak is MI355X's FP64 vector "
+                    "78.6 TFLOP/s; flops by the kernel's own Newton/state iteration counts",
+            "flops_per_launch": flops, "cells_per_launch": cells,
+            "hbm_gbs_algorithmic": RESPONSE_BYTES_PER_CELL * cells / avg_s / 1e9,
+            "traffic": recorded_traffic("response")}
 
 
-def cpu_baseline(args, loads):
-    """The CPU oracle's full solver on a bounded sample of the same workload:
-    a 32^3 grid with the same grain size (grains scaled by the volume) through
-    the first increments of the same load case."""
-    lib = Library(ORACLE_SO)
-    c = args.cpu_sample_cells
-    grains = max(2, round(args.grains * (c / args.cells) ** 3))
+def run_batched_workload(args, precision: str, batched: bool, workers: int):
+    from solver_cpp_amd.serving import batched_specs, run_batched
+
+    lib = Library(PRODUCT_SO, boundary=True)
+    prec = 1 if precision == "mixed" else 0
+    specs = batched_specs(lib, args.batched_rves, seed=7, precision=prec, **EXAMPLE_NUMERICS)
+    t0 = time.perf_counter()
+    r = run_batched(lib, specs, workers, int(200 * (1 << 30)), t=5.0, N=10, batched=batched, max_wait_ms=5.0)
+    log(f"[bench] batched workload ({precision}, {'joint launches' if batched else 'one at a time'}): "
+        f"{r['value']:.3e} updates/s, {r['wall']:.1f} s, batches {r['batches']}, max batch {r['max_batch']}, "
+        f"failed {r['failed']}")
+    out = {"value": r["value"], "wall_s": r["wall"], "updates": r["updates"], "failed_increments": r["failed"],
+           "p50_increment_ms": float(np.percentile(r["increment_ms"], 50)),
+           "p99_increment_ms": float(np.percentile(r["increment_ms"], 99)),
+           "peak_gb": r["peak_bytes"] / 2 ** 30}
+    if batched:
+        out.update(batches=r["batches"], max_batch=r["max_batch"],
+                   mean_batch=r["batched_evaluations"] / max(r["batches"], 1))
+    P = [np.array(x.P_av) for rs in r["results"] for x in rs]
+    del specs
+    return out, P
+
+
+def cpu_info():
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        model = platform.processor()
+    nproc = os.cpu_count() or 1
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except Exception:
+        pass
+    return model, nproc
+
+
+def _omp_threads(n: int):
+    """Thread count of the oracle's OpenMP loops (the material point)."""
+    try:
+        C.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+    except OSError:
+        pass
+
+
+def _oracle_sample(lib, c, grains, args, loads, n_inc):
     mat, ori, size = polycrystal((c, c, c), grains, seed=args.seed, ori_seed=args.ori_seed)
     rve = Rve(lib, (c, c, c), size, mat, ori, **EXAMPLE_NUMERICS)
     rve.init()
     it = rve.increments(loads)
     t0 = time.perf_counter()
-    evals, n_inc = 0, 0
-    while n_inc < args.cpu_sample_increments:
+    evals = 0
+    for _ in range(n_inc):
         r = next(it)
         evals += r["evaluations"]
-        n_inc += 1
-        log(f"[bench] cpu oracle increment {r['increment']}: {r['seconds']:.2f} s, {r['evaluations']} evaluations")
     wall = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": c ** 3 * evals / wall, "unit": "mat-point updates/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (FP64 C++ restatement, OpenMP) full solver, {c}^3 grid, {grains} grains, "
-                      f"first {n_inc} increments of the same load case ({evals} residual evaluations, {wall:.1f} s)"}
+    rve.close()
+    return c ** 3 * evals / wall, evals, wall
+
+
+def cpu_baseline(args, loads):
+    """The CPU oracle (the FP64 C++ restatement, OpenMP over material points)
+    on bounded samples of the same workload, with the same grain size (grains
+    scaled by the volume) through the first increments of the same load case:
+    all threads on a 32^3 sample (the reported value), one thread on a 16^3
+    sample, and one residual evaluation of the 128^3 workload itself (all
+    threads) from which the full run's CPU time is extrapolated."""
+    lib = Library(ORACLE_SO)
+    model, nproc = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", nproc))
+
+    def grains_for(c):
+        return max(2, round(args.grains * (c / args.cells) ** 3))
+
+    _omp_threads(threads)
+    v_all, e_all, w_all = _oracle_sample(lib, args.cpu_sample_cells, grains_for(args.cpu_sample_cells), args, loads,
+                                         args.cpu_sample_increments)
+    log(f"[bench] cpu oracle {threads} threads, {args.cpu_sample_cells}^3: {v_all:.3e} updates/s ({w_all:.1f} s)")
+    _omp_threads(1)
+    v_one, e_one, w_one = _oracle_sample(lib, 16, grains_for(16), args, loads, args.cpu_sample_increments)
+    log(f"[bench] cpu oracle 1 thread, 16^3: {v_one:.3e} updates/s ({w_one:.1f} s)")
+    _omp_threads(threads)
+    out = {"value": v_all, "unit": "mat-point updates/s", "cores": threads, "kind": "port",
+           "sample": f"oracle (FP64 C++ restatement; OpenMP over material points, FFTs serial) full solver, "
+                     f"{args.cpu_sample_cells}^3 grid, {grains_for(args.cpu_sample_cells)} grains, first "
+                     f"{args.cpu_sample_increments} increments of the same load case ({e_all} residual "
+                     f"evaluations, {w_all:.1f} s)",
+           "one_thread": {"value": v_one, "sample": f"16^3 grid, {grains_for(16)} grains, first "
+                                                    f"{args.cpu_sample_increments} increments ({e_one} evaluations, "
+                                                    f"{w_one:.1f} s)"},
+           "cpu_model": model, "nproc": nproc}
+    if args.cpu_128:
+        c = args.cells
+        mat, ori, size = polycrystal((c, c, c), args.grains, seed=args.seed, ori_seed=args.ori_seed)
+        rve = Rve(lib, (c, c, c), size, mat, ori, **EXAMPLE_NUMERICS)
+        rve.init()
+        rve.forward(False, False, loads[0].t / loads[0].N, 1.0, loads[0].t, loads[0])
+        F = rve.field("F")
+        t0 = time.perf_counter()
+        rve.form_residual(F)
+        t_eval = time.perf_counter() - t0
+        rve.close()
+        out["c3_evaluation"] = {"seconds": t_eval, "updates_per_s": c ** 3 / t_eval,
+                                "extrapolation": "full 60-increment run = its residual evaluations (GPU count) x "
+                                                 "this time"}
+        log(f"[bench] cpu oracle one 128^3 evaluation: {t_eval:.1f} s")
+    return out
 
 
 def main():
@@ -175,6 +270,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cells", type=int, default=32)
     ap.add_argument("--cpu-sample-increments", type=int, default=2)
+    ap.add_argument("--cpu-128", action="store_true", help="also time one 128^3 residual evaluation on the CPU")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    ap.add_argument("--batched-rves", type=int, default=256)
+    ap.add_argument("--no-tight", action="store_true", help="skip the tight-tolerance precision comparison")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -191,8 +290,8 @@ def main():
     mat, ori, size = polycrystal(cells, args.grains, seed=args.seed, ori_seed=args.ori_seed)
     plain = run_gpu(args, loads, mat, ori, size, "fp64")
     res = plain if args.precision == "fp64" else run_gpu(args, loads, mat, ori, size, args.precision)
-    value = res["N"] * res["evals"] / res["wall"]
-    plain_value = plain["N"] * plain["evals"] / plain["wall"]
+    value = res["updates"] / res["wall"]
+    plain_value = plain["updates"] / plain["wall"]
     lat_ms = np.array(res["lat"]) * 1e3
     plain_ms = np.array(plain["lat"]) * 1e3
     line = {
@@ -222,19 +321,50 @@ def main():
         "plain_fp64": {"value": plain_value, "ms_per_step": float(plain_ms.mean()),
                        "p50_increment_ms": float(np.percentile(plain_ms, 50)),
                        "p99_increment_ms": float(np.percentile(plain_ms, 99)),
-                       "residual_evaluations": plain["evals"]},
+                       "residual_evaluations": plain["evals"], "material_point_updates": plain["updates"]},
         "speedup_vs_plain_fp64": value / plain_value,
+        # the equal-work comparison: both forms solve the same increments
         "increment_time_ratio_plain_over_this": float(plain_ms.mean() / lat_ms.mean()),
         "error_vs_plain_fp64": {
             "P_av_rel": float(np.abs(res["P_av"] - plain["P_av"]).max() / np.abs(plain["P_av"]).max()),
             "P_max_rel": float(np.abs(res["P"] - plain["P"]).max() / np.abs(plain["P"]).max())},
         "fp32_evaluations": int(res["stats"].fp32_evaluations),
         "residual_evaluations": res["evals"],
+        "material_point_updates": res["updates"],
         "newton_iterations": int(res["stats"].newton_iterations),
         "state_iterations": int(res["stats"].state_iterations),
         "kernels_ms": {k: round(v["ms"], 3) for k, v in res["kt"].items() if v["launches"]},
         "roofline": roofline(res),
     }
+    del plain, res
+    if args.precision == "mixed" and not args.no_tight:
+        # precision error apart from the stopping spread: both forms with the
+        # divergence and stress tolerances 1000 times tighter
+        tight = dict(itmax=100, eps_rel_div=5e-7, eps_abs_div=1e-12, eps_rel_P=1e-6, eps_abs_P=1e-6)
+        targs = argparse.Namespace(**vars(args))
+        targs.steps, targs.warmup = 2, 1
+        tp = run_gpu(targs, loads, mat, ori, size, "fp64", **tight)
+        tm = run_gpu(targs, loads, mat, ori, size, "mixed", **tight)
+        line["error_vs_plain_fp64_tight_tolerances"] = {
+            "P_av_rel": float(np.abs(tm["P_av"] - tp["P_av"]).max() / np.abs(tp["P_av"]).max()),
+            "P_max_rel": float(np.abs(tm["P"] - tp["P"]).max() / np.abs(tp["P"]).max()),
+            "tolerances": tight, "increments": [2, 3]}
+        del tp, tm
+    if not args.no_batched:
+        bm, Pm = run_batched_workload(args, "mixed", True, args.batched_rves)
+        bf, Pf = run_batched_workload(args, "fp64", True, args.batched_rves)
+        one, Po = run_batched_workload(args, "fp64", False, 1)
+        same = all(np.array_equal(a, b) for a, b in zip(Pf, Po))
+        err = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(Pm, Po))
+        line["batched"] = {
+            "workload": f"{args.batched_rves} RVEs (generator seed 7): cubic grids with edges uniform in "
+                        f"{{16,24,32,48}}, 10-200 Voronoi grains, tension along uniform random directions, 10 "
+                        f"increments of 0.5 s each; one material-point launch per evaluation for all running RVEs",
+            "mixed": bm, "fp64": bf, "plain_fp64_one_at_a_time": one,
+            "speedup_mixed_vs_plain_one_at_a_time": bm["value"] / one["value"],
+            "speedup_fp64_vs_plain_one_at_a_time": bf["value"] / one["value"],
+            "fp64_bitwise_equal_to_one_at_a_time": bool(same),
+            "mixed_max_P_av_rel_diff_vs_fp64": err}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args, loads)
     print(json.dumps(line), flush=True)

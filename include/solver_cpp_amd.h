@@ -67,6 +67,9 @@ typedef struct sc_grid_numerics {
   int    stag_iter_max;           /* 10 */
   int    max_cut_back;            /* 3 */
   int    ngmres_m;                /* NGMRES history size (PETSc default 30) */
+  int    fft_plan_mode;           /* FFT.FFTW_plan_mode (config.cpp:432-441): 0 estimate (heuristic
+                                     launch shapes), 1 measure (default: launch shapes timed per grid;
+                                     patient and exhaustive map here).  Shapes change no result. */
 } sc_grid_numerics;
 
 /* Arithmetic of the spectral stages. */
@@ -304,11 +307,24 @@ typedef struct sc_server_stats {
   int64_t resident_jobs, completed_requests;
   int     max_concurrent;         /* most requests running at once */
   int64_t evictions;              /* idle resident jobs checkpointed to host memory */
+  int64_t batches;                /* joint launches of residual evaluations (batched server) */
+  int64_t batched_evaluations;    /* residual evaluations computed in them */
+  int64_t max_batch;              /* most evaluations in one joint launch */
 } sc_server_stats;
 
 /* workers: host threads, each driving one admitted job at a time on its own
  * HIP stream; hbm_budget_bytes bounds the sum of resident jobs' footprints. */
 int  sc_server_create(int64_t hbm_budget_bytes, int workers, sc_server** out);
+/* As sc_server_create, and every residual evaluation of the jobs that are
+ * inside an increment is computed in joint launches on one stream: the
+ * material-point update of all their cells in one launch, the P sums and the
+ * aims in one each, and per group of jobs with the same grid and arithmetic
+ * one gather, one batched FFT per pass, one Gamma convolution and one
+ * scatter.  A joint launch starts when every running job waits for an
+ * evaluation or the oldest request has waited max_wait_ms.  Each job's
+ * results are bitwise those of sc_server_create.  For throughput over many
+ * RVEs, workers should be about the number of jobs run at once. */
+int  sc_server_create_batched(int64_t hbm_budget_bytes, int workers, double max_wait_ms, sc_server** out);
 void sc_server_destroy(sc_server* server);  /* finishes running requests */
 /* A job: RVE description (copied) and its load case.  keep_resident = 0
  * releases the job's memory as soon as it has no pending request. */
@@ -319,6 +335,9 @@ int  sc_server_submit(sc_server* server, int64_t job, int64_t* request);
 /* Blocks until done (timeout_s < 0: no limit; returns 4 on timeout). */
 int  sc_server_wait(sc_server* server, int64_t request, double timeout_s, sc_request_result* out);
 int  sc_server_remove_job(sc_server* server, int64_t job);
+/* A field of a resident job with no request running or pending, in the
+ * layouts of sc_rve_get_field (SC_FIELD_*). */
+int  sc_server_get_field(sc_server* server, int64_t job, int field, double* out, int64_t n_values);
 int  sc_server_get_stats(sc_server* server, sc_server_stats* out);
 
 /* ------------------------------------------------------------------------- */

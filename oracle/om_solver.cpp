@@ -502,6 +502,7 @@ void sc_default_grid_numerics(sc_grid_numerics* g) {
   g->stag_iter_max = 10;
   g->max_cut_back = 3;
   g->ngmres_m = 30;
+  g->fft_plan_mode = 1;  // no planner here: any mode computes the same
 }
 
 int sc_rve_create(const sc_rve_desc* d, sc_rve** out) {

@@ -42,6 +42,7 @@ class GridNumerics(C.Structure):
         ("eps_abs_P", C.c_double), ("eps_rel_P", C.c_double),
         ("divergence_correction", C.c_int), ("derivative", C.c_int),
         ("stag_iter_max", C.c_int), ("max_cut_back", C.c_int), ("ngmres_m", C.c_int),
+        ("fft_plan_mode", C.c_int),
     ]
 
 
@@ -103,7 +104,8 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64), ("peak_bytes", C.c_int64),
         ("resident_jobs", C.c_int64), ("completed_requests", C.c_int64), ("max_concurrent", C.c_int),
-        ("evictions", C.c_int64),
+        ("evictions", C.c_int64), ("batches", C.c_int64), ("batched_evaluations", C.c_int64),
+        ("max_batch", C.c_int64),
     ]
 
 
@@ -214,12 +216,14 @@ BOUNDARY_SIGNATURES = {
     "sc_read_load": (C.c_int, [C.c_char_p, C.POINTER(LoadStep), C.c_int, _ip]),
     "sc_rve_footprint": (C.c_int64, [C.POINTER(RveDesc)]),
     "sc_server_create": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_void_p)]),
+    "sc_server_create_batched": (C.c_int, [C.c_int64, C.c_int, C.c_double, C.POINTER(C.c_void_p)]),
     "sc_server_destroy": (None, [C.c_void_p]),
     "sc_server_add_job": (C.c_int, [C.c_void_p, C.POINTER(RveDesc), C.POINTER(LoadStep), C.c_int, C.c_int,
                                     C.POINTER(C.c_int64)]),
     "sc_server_submit": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
     "sc_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.c_double, C.POINTER(RequestResult)]),
     "sc_server_remove_job": (C.c_int, [C.c_void_p, C.c_int64]),
+    "sc_server_get_field": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _dp, C.c_int64]),
     "sc_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
 }
 

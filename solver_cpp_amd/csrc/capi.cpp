@@ -216,6 +216,7 @@ void sc_default_grid_numerics(sc_grid_numerics* g) {
   g->stag_iter_max = 10;
   g->max_cut_back = 3;
   g->ngmres_m = 30;
+  g->fft_plan_mode = 1;
 }
 
 // ---- RVE context -------------------------------------------------------------
@@ -443,8 +444,14 @@ int sc_rve_form_residual(sc_rve* R, const double* F, double* r) {
 }
 
 int sc_rve_get_field(sc_rve* R, int field, double* out, int64_t n) {
-  return guarded([&] {
-    sca::Rve& v = *R->r;
+  return guarded([&] { return sca::rve_get_field(*R->r, field, out, n); });
+}
+
+}  // extern "C"
+
+// host copy of a field in the reference's layout (sc_rve_get_field)
+int sca::rve_get_field(sca::Rve& v, int field, double* out, int64_t n) {
+  {
     const int64_t N = v.N;
     const double* src = nullptr;
     int ncomp = 9;
@@ -487,8 +494,10 @@ int sc_rve_get_field(sc_rve* R, int field, double* out, int64_t n) {
     a.get(out, v.st);
     SCA_HIP(hipStreamSynchronize(v.st));
     return 0;
-  });
+  }
 }
+
+extern "C" {
 
 int sc_rve_set_field(sc_rve* R, int field, const double* in, int64_t n) {
   return guarded([&] {

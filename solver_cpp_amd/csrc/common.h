@@ -69,12 +69,67 @@ struct MpState {
   int* repivot = nullptr;    // (1 + N): list of the points the pivoting pass redoes (count first)
 };
 
+// Argument tables of batched launches: pinned host staging and a device
+// buffer.  reserve() (before a batch, may reallocate) then add() slots; each
+// launcher fills its slots and uploads them on its stream.  A batch's
+// tables are rewritten only after the previous batch completed.
+class TableArena {
+ public:
+  TableArena() = default;
+  TableArena(const TableArena&) = delete;
+  TableArena& operator=(const TableArena&) = delete;
+  ~TableArena();
+  void reserve(size_t bytes);  // and empties the arena
+  // n entries of E: host slot (returned) and its device address (*dev)
+  template <typename E>
+  E* add(int n, E** dev) {
+    const size_t off = used;
+    used = (used + (size_t)n * sizeof(E) + 255) & ~(size_t)255;
+    if (used > cap) throw std::runtime_error("table arena too small");
+    *dev = reinterpret_cast<E*>(dptr + off);
+    return reinterpret_cast<E*>(hptr + off);
+  }
+  template <typename E>
+  void upload(const E* host, int n, hipStream_t st) {
+    const size_t off = reinterpret_cast<const char*>(host) - hptr;
+    SCA_HIP(hipMemcpyAsync(dptr + off, host, (size_t)n * sizeof(E), hipMemcpyHostToDevice, st));
+  }
+
+ private:
+  char *hptr = nullptr, *dptr = nullptr;
+  size_t cap = 0, used = 0;
+};
+
+// One RVE's share of a batched material-point update
+// (launch_mp_response_batch): its planes with s.F the F of this evaluation,
+// its material, time step, terminally-ill flag, substep scratch and the 9 x
+// mp_partial_blocks(N) doubles that receive the block sums of P.
+struct MpBatchItem {
+  MpState s;
+  MatConst mc;
+  double dt;
+  int* ill;
+  double* sub;
+  double* pav_partial;
+};
+// P block sums of one RVE of a batch (device table entry)
+struct MpBatchPav {
+  const double* P;
+  int64_t N;
+  double* partial;
+  int nb;
+};
+
 // Kernel launchers (constitutive.hip)
 // cells [c_begin, c_end) of the planes (c_end < 0: all N); *d_ill is cleared
 // by the launch and set when a point is terminally ill
 void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_ill, double* d_sub, hipStream_t st,
                         int* d_broken = nullptr, int64_t c_begin = 0, int64_t c_end = -1);
 int mp_partial_blocks(int64_t N);
+// Material-point updates of n RVEs in one launch (and their pivoting passes
+// and P block sums), each RVE's results bitwise those of launch_mp_response
+// + launch_pav_partial; the argument tables go into ta.
+void launch_mp_response_batch(const MpBatchItem* items, int n, TableArena& ta, hipStream_t st);
 int mp_sub_planes();
 void launch_pav_partial(const double* P, int64_t N, double* partial, hipStream_t st);
 void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF /*(81,N) SoA*/,

@@ -901,20 +901,79 @@ __global__ void repivot_reset_kernel(int* repivot, int* ill) {
   *ill = 0;
 }
 
+// ---- many RVEs in one launch ---------------------------------------------------
+// The material-point updates of a batch of RVEs (each with its own planes,
+// material, time step and F) as one launch: block b belongs to the RVE r with
+// blk_off[r] <= b < blk_off[r + 1] and takes its cells (b - blk_off[r]) * 64
+// on.  Every point runs exactly the code of mp_response_kernel, so each RVE's
+// results are bitwise those of its own launch.  The table is read only
+// (restrict): its fields are loaded as scalars and never reloaded after the
+// kernel's stores.
+__device__ __forceinline__ int batch_entry(const int* __restrict__ blk_off, int n, unsigned b) {
+  int lo = 0, hi = n;  // blk_off[lo] <= b < blk_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((unsigned)blk_off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+// A table entry read through the constant address space: the loads are
+// invariant for the kernel, as kernel arguments are, so the compiler may
+// reload any field where it is used instead of keeping it in registers.
+__device__ __forceinline__ KernelArgs load_args(const KernelArgs* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(4))) const KernelArgs ConstArgs;
+  return *(const ConstArgs*)p;
+#else
+  return *p;
+#endif
+}
+
+template <int NE4, int AE4>
+__global__ void __launch_bounds__(64) mp_response_batch_kernel(const KernelArgs* __restrict__ tab,
+                                                                const int* __restrict__ blk_off, int n) {
+  __shared__ MpLds sh;
+  const int r = batch_entry(blk_off, n, blockIdx.x);
+  const KernelArgs a = load_args(tab + r);
+  const int64_t c = a.c_begin + (int64_t)(blockIdx.x - (unsigned)blk_off[r]) * 64 + threadIdx.x;
+  if (c >= a.c_end) return;
+  mp_point<NE4, AE4, false>(a, c, sh);
+}
+
+// pivoting pass of a batch: blockIdx.y is the RVE, blockIdx.x strides over
+// its list as mp_repivot_kernel does
+template <int NE4, int AE4>
+__global__ void __launch_bounds__(64) mp_repivot_batch_kernel(const KernelArgs* __restrict__ tab) {
+  __shared__ MpLds sh;
+  const KernelArgs a = load_args(tab + blockIdx.y);
+  const int64_t cnt = a.repivot[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x)
+    mp_point<NE4, AE4, true>(a, a.repivot[1 + i], sh);
+}
+
+__global__ void repivot_reset_batch_kernel(const KernelArgs* __restrict__ tab, int n) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  tab[r].repivot[0] = 0;
+  *tab[r].ill = 0;
+}
+
 
 // Block partial sums of P for P_av (deterministic): block b sums cells
 // [b*PAV_CELLS, (b+1)*PAV_CELLS) of each of the 9 planes, every thread its
 // PAV_CELLS/256 cells at stride 256 in order, then one fixed tree over the
 // 256 threads for all 9 sums at once.
 constexpr int PAV_CELLS = 2048;
-__global__ void __launch_bounds__(256) pav_partial_kernel(const double* P, int64_t N, double* partial) {
-  __shared__ double red[9][256];
+__device__ __forceinline__ void pav_block(const double* P, int64_t N, double* partial, int nb, unsigned blk,
+                                          double (*red)[256]) {
   const int t = threadIdx.x;
-  const int64_t c0 = (int64_t)blockIdx.x * PAV_CELLS + t;
+  const int64_t c0 = (int64_t)blk * PAV_CELLS + t;
   double acc[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) acc[q] = 0.0;
-  if ((int64_t)(blockIdx.x + 1) * PAV_CELLS <= N) {
+  if ((int64_t)(blk + 1) * PAV_CELLS <= N) {
     // interior block: no per-element test, so every load can be issued
     // ahead of the sums (same summation order)
 #pragma unroll
@@ -943,7 +1002,21 @@ __global__ void __launch_bounds__(256) pav_partial_kernel(const double* P, int64
     }
     __syncthreads();
   }
-  if (t < 9) partial[(int64_t)t * gridDim.x + blockIdx.x] = red[t][0];
+  if (t < 9) partial[(int64_t)t * nb + blk] = red[t][0];
+}
+
+__global__ void __launch_bounds__(256) pav_partial_kernel(const double* P, int64_t N, double* partial) {
+  __shared__ double red[9][256];
+  pav_block(P, N, partial, gridDim.x, blockIdx.x, red);
+}
+
+// P block sums of a batch: the blocks of RVE r follow blk_off[r], each as in
+// its own launch (same cells, same order)
+__global__ void __launch_bounds__(256) pav_partial_batch_kernel(const MpBatchPav* __restrict__ tab,
+                                                                 const int* __restrict__ blk_off, int n) {
+  __shared__ double red[9][256];
+  const int r = batch_entry(blk_off, n, blockIdx.x);
+  pav_block(tab[r].P, tab[r].N, tab[r].partial, tab[r].nb, blockIdx.x - (unsigned)blk_off[r], red);
 }
 
 // ---------------------------------------------------------------------------
@@ -1263,6 +1336,76 @@ void launch_mp_response(const MpState& s, const MatConst& mc, double dt, int* d_
     hipLaunchKernelGGL((mp_response_kernel<-1, -1>), dim3(blocks), dim3(64), 0, st, a);
     hipLaunchKernelGGL((mp_repivot_kernel<-1, -1>), dim3(blocks2), dim3(64), 0, st, a);
   }
+  SCA_HIP(hipGetLastError());
+}
+
+// Batched update: the KernelArgs of every item grouped by exponent class
+// (class 0: the example exponents, compiled in; class 1: any other), the
+// block offsets of each class, and the P block-sum entries with theirs.
+void launch_mp_response_batch(const MpBatchItem* items, int n, TableArena& ta, hipStream_t st) {
+  if (n <= 0) return;
+  KernelArgs* dka = nullptr;
+  int *doff = nullptr, *dpvoff = nullptr;
+  MpBatchPav* dpv = nullptr;
+  KernelArgs* ka = ta.add<KernelArgs>(n, &dka);
+  int* off = ta.add<int>(n + 2, &doff);
+  MpBatchPav* pv = ta.add<MpBatchPav>(n, &dpv);
+  int* pvoff = ta.add<int>(n + 1, &dpvoff);
+  auto cls = [](const MatConst& mc) { return (mc.n_e4 == 80 && mc.a_e4 == 9) ? 0 : 1; };
+  int n0 = 0;
+  for (int i = 0; i < n; ++i) n0 += cls(items[i].mc) == 0 ? 1 : 0;
+  int k0 = 0, k1 = n0;
+  int64_t maxN = 0;
+  for (int i = 0; i < n; ++i) {
+    const MpBatchItem& it = items[i];
+    const MpState& s = it.s;
+    if (s.N > ((int64_t)1 << 29)) throw std::runtime_error("material-point planes longer than 2^29 cells");
+    if (!s.repivot || !it.ill || !it.sub || !it.pav_partial) throw std::runtime_error("incomplete batch item");
+    const int e = cls(it.mc) == 0 ? k0++ : k1++;
+    ka[e] = KernelArgs{s, it.mc, it.dt, it.ill, nullptr, s.repivot, it.sub, 0, s.N};
+    pv[e] = MpBatchPav{s.P, s.N, it.pav_partial, mp_partial_blocks(s.N)};
+    maxN = std::max(maxN, s.N);
+  }
+  // block offsets: class 0 in off[0..n0], class 1 in off[n0+1..n+1]
+  int64_t tot = 0;
+  off[0] = 0;
+  for (int i = 0; i < n0; ++i) {
+    tot += (ka[i].c_end + 63) / 64;
+    off[i + 1] = (int)tot;
+  }
+  const int64_t blocks0 = tot;
+  tot = 0;
+  off[n0 + 1] = 0;
+  for (int i = n0; i < n; ++i) {
+    tot += (ka[i].c_end + 63) / 64;
+    off[i + 2] = (int)tot;
+  }
+  const int64_t blocks1 = tot;
+  tot = 0;
+  pvoff[0] = 0;
+  for (int i = 0; i < n; ++i) {
+    tot += pv[i].nb;
+    pvoff[i + 1] = (int)tot;
+  }
+  const int64_t pav_blocks = tot;
+  if (blocks0 + blocks1 > INT32_MAX || pav_blocks > INT32_MAX) throw std::runtime_error("batch too large");
+  ta.upload(ka, n, st);
+  ta.upload(off, n + 2, st);
+  ta.upload(pv, n, st);
+  ta.upload(pvoff, n + 1, st);
+  hipLaunchKernelGGL(repivot_reset_batch_kernel, dim3((n + 63) / 64), dim3(64), 0, st, dka, n);
+  // pivoting pass: enough lanes per RVE for its list (about 1 % of its points)
+  const int x2 = (int)std::min<int64_t>(1024, std::max<int64_t>(16, (maxN + 1023) / 1024));
+  if (n0 > 0) {
+    hipLaunchKernelGGL((mp_response_batch_kernel<80, 9>), dim3((unsigned)blocks0), dim3(64), 0, st, dka, doff, n0);
+    hipLaunchKernelGGL((mp_repivot_batch_kernel<80, 9>), dim3(x2, n0), dim3(64), 0, st, dka);
+  }
+  if (n > n0) {
+    hipLaunchKernelGGL((mp_response_batch_kernel<-1, -1>), dim3((unsigned)blocks1), dim3(64), 0, st, dka + n0,
+                       doff + n0 + 1, n - n0);
+    hipLaunchKernelGGL((mp_repivot_batch_kernel<-1, -1>), dim3(x2, n - n0), dim3(64), 0, st, dka + n0);
+  }
+  hipLaunchKernelGGL(pav_partial_batch_kernel, dim3((unsigned)pav_blocks), dim3(256), 0, st, dpv, dpvoff, n);
   SCA_HIP(hipGetLastError());
 }
 

@@ -10,9 +10,11 @@
 // stage that is not the last writes its outputs to LDS at the Stockham
 // positions and the next stage reads t + TP * j back; the last stage's
 // outputs land on the thread's own positions, so the first stage reads and
-// the last stage writes global memory directly, coalesced.  A 256-thread
-// block holds P = 256 / TP pencils.  The twiddle table (length N) is staged
-// in LDS with the data.
+// the last stage writes global memory directly, coalesced.  A block of BS
+// threads (a power of two from TP to 512, chosen per plan: 256 by default,
+// or by the plan's autotuner) holds P = BS / TP pencils; the block size
+// changes no arithmetic.  The twiddle table (length N) is staged in LDS
+// with the data.
 //
 // Supported: N = 2^a (8..512, E = 8) and N = 3 * 2^a (12..384, E = 12);
 // other lengths use the generic Stockham kernels.
@@ -188,10 +190,11 @@ template <typename T, int N>
 struct Shape {
   static constexpr int E = points_per_thread_t<T>(N);
   static constexpr int TP = N / E;     // threads per pencil
-  static constexpr int P = 256 / TP;   // pencils per 256-thread block
   static constexpr int LD = N + 1;     // padded LDS row
-  static constexpr int LDS = P * LD + N;  // complex entries: rows + twiddles
+  // complex LDS entries of a block of bs threads: rows + twiddles
+  static constexpr int lds(int bs) { return bs / TP * LD + N; }
 };
+constexpr int MAX_PENCIL_BLOCK = 512;
 
 template <typename T, int N>
 __device__ __forceinline__ const cpx<T>* stage_tw(cpx<T>* lds_tw, const cpx<T>* tw) {
@@ -203,14 +206,15 @@ __device__ __forceinline__ const cpx<T>* stage_tw(cpx<T>* lds_tw, const cpx<T>* 
 // (o / d1) * s1 + (o % d1) * s2 + i with element stride es; a block takes P
 // consecutive i of one o, the pencil index fastest across lanes
 template <typename T, int N, int SIGN>
-__global__ void __launch_bounds__(256) strided_kernel(cpx<T>* data, int ni, int d1, int64_t s1, int64_t s2,
-                                                      int64_t es, const cpx<T>* tw_g) {
+__global__ void __launch_bounds__(MAX_PENCIL_BLOCK) strided_kernel(cpx<T>* data, int ni, int d1, int64_t s1,
+                                                                   int64_t s2, int64_t es, const cpx<T>* tw_g) {
   using S = Shape<T, N>;
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
-  const cpx<T>* tw = stage_tw<T, N>(lds + S::P * S::LD, tw_g);
-  const int p = threadIdx.x % S::P, t = threadIdx.x / S::P;
-  const int i = blockIdx.x * S::P + p;
+  const int P = blockDim.x / S::TP;  // a power of two
+  const cpx<T>* tw = stage_tw<T, N>(lds + P * S::LD, tw_g);
+  const int p = threadIdx.x & (P - 1), t = threadIdx.x >> __builtin_ctz(P);
+  const int i = blockIdx.x * P + p;
   const int64_t o = blockIdx.y;
   const int64_t base = (o / d1) * s1 + (o % d1) * s2 + i;
   const bool ok = i < ni;
@@ -228,14 +232,16 @@ __global__ void __launch_bounds__(256) strided_kernel(cpx<T>* data, int ni, int 
 // (z = x_a + i x_b): X_a[k] = (Z[k] + conj Z[N-k]) / 2,
 // X_b[k] = (Z[k] - conj Z[N-k]) / (2 i)
 template <typename T, typename TI, int N>
-__global__ void __launch_bounds__(256) x_r2c_kernel(const TI* in, cpx<T>* out, int64_t nrows, const cpx<T>* tw_g) {
+__global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, cpx<T>* out, int64_t nrows,
+                                                                 const cpx<T>* tw_g) {
   using S = Shape<T, N>;
   constexpr int NR = N / 2 + 1;
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
-  const cpx<T>* tw = stage_tw<T, N>(lds + S::P * S::LD, tw_g);
+  const int P = blockDim.x / S::TP;
+  const cpx<T>* tw = stage_tw<T, N>(lds + P * S::LD, tw_g);
   const int t = threadIdx.x % S::TP, p = threadIdx.x / S::TP;
-  const int64_t ra = 2 * ((int64_t)blockIdx.x * S::P + p), rb = ra + 1;
+  const int64_t ra = 2 * ((int64_t)blockIdx.x * P + p), rb = ra + 1;
   cpx<T>* row = lds + p * S::LD;
   cpx<T> v[S::E];
 #pragma unroll
@@ -260,15 +266,16 @@ __global__ void __launch_bounds__(256) x_r2c_kernel(const TI* in, cpx<T>* out, i
 // x pass, half complex -> real (Hermitian completion; the imaginary parts of
 // the real modes k = 0 and k = N/2 are dropped), scaled; two rows per pencil
 template <typename T, typename TO, int N>
-__global__ void __launch_bounds__(256) x_c2r_kernel(const cpx<T>* in, TO* out, int64_t nrows, const cpx<T>* tw_g,
-                                                    T scale) {
+__global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_c2r_kernel(const cpx<T>* in, TO* out, int64_t nrows,
+                                                                 const cpx<T>* tw_g, T scale) {
   using S = Shape<T, N>;
   constexpr int NR = N / 2 + 1;
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
-  const cpx<T>* tw = stage_tw<T, N>(lds + S::P * S::LD, tw_g);
+  const int P = blockDim.x / S::TP;
+  const cpx<T>* tw = stage_tw<T, N>(lds + P * S::LD, tw_g);
   const int t = threadIdx.x % S::TP, p = threadIdx.x / S::TP;
-  const int64_t ra = 2 * ((int64_t)blockIdx.x * S::P + p), rb = ra + 1;
+  const int64_t ra = 2 * ((int64_t)blockIdx.x * P + p), rb = ra + 1;
   cpx<T> v[S::E];
 #pragma unroll
   for (int j = 0; j < S::E; ++j) {

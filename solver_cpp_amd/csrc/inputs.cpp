@@ -584,6 +584,12 @@ void read_numerics(const std::string& text, sc_grid_numerics& g, sc_phase_numeri
           else if (v->value == "size+grid" || v->value == "grid+size") g.divergence_correction = 2;
           else throw std::runtime_error("numerics: divergence_correction " + v->value);
         }
+        if (const Node* v = fft->get("FFTW_plan_mode")) {
+          if (v->value == "fftw_estimate") g.fft_plan_mode = 0;
+          else if (v->value == "fftw_measure" || v->value == "fftw_patient" || v->value == "fftw_exhaustive")
+            g.fft_plan_mode = 1;
+          else throw std::runtime_error("numerics: FFTW_plan_mode " + v->value);
+        }
         if (const Node* v = fft->get("derivative")) {
           if (v->value == "continuous") g.derivative = 0;
           else if (v->value == "central_difference") g.derivative = 1;

@@ -14,11 +14,19 @@ struct sc_rve;
 // grids up to this many cells evaluate their residual as one HIP graph
 constexpr int64_t RESIDUAL_GRAPH_MAX_CELLS = 64 * 64 * 64;
 
+// mixed precision: the FP32 stage's relative divergence tolerance is never
+// tighter than this (about 170 FP32 rounding units)
+constexpr double MIXED_FP32_REL_DIV_MIN = 1.0e-5;
+
 namespace sca {
 
+class Batcher;
 extern thread_local std::string g_err;
 
 void masked_compliance_host(const double* C, const double q[4], const int mask[9], double* S_out);
+// throws std::runtime_error naming the first problem of a description that
+// a context cannot be built from (checked before anything is allocated)
+void validate_rve_desc(const sc_rve_desc& d);
 void sym_lstsq(int l, std::vector<double> H, std::vector<double>& b);
 
 // offsets into the small device scratch (doubles); [0, MAX_RED_VEC) holds
@@ -75,6 +83,8 @@ class Ngmres {
   // called after every synchronisation of the reductions (the owner takes up
   // results it copied asynchronously)
   std::function<void()> after_sync;
+  // the point a solve returned was the point of its last evaluation
+  bool solution_evaluated_last = false;
   int64_t n;
   hipStream_t st;
   double *partial = nullptr, *small = nullptr;
@@ -117,6 +127,7 @@ class Rve {
   int next_increment(const sc_loadstep* steps, int n_steps, LoadCursor& cur, int* iterations, int* evaluations);
   void form_residual(const double* Fdev, double* rdev);
   void launch_residual(const double* Fdev, double* rdev);
+  void launch_spectral(double* rdev);
   bool ensure_residual_graph();  // false: evaluate eagerly this time
   void release_residual_graph();
   void constitutive_response(const double* Fdev, double dt);
@@ -139,6 +150,11 @@ class Rve {
   // (damask_grid.cpp:165-168)
   std::function<void(int, double)> on_result;
 
+  // residual evaluations are computed in joint launches with other contexts
+  // (null: on this context's own stream)
+  Batcher* batch = nullptr;
+  GammaHost gamma_host(bool do_gamma) const;
+
   // descriptor and geometry
   sc_rve_desc desc;
   std::vector<int> material_at;
@@ -156,6 +172,9 @@ class Rve {
   bool mixed = false;
   bool use32 = false;     // the current residual evaluations use the FP32 stages
   int iter_base = 0;      // total_iter offset of an FP64 finishing solve
+  // the next evaluation takes the material point as last updated (at the
+  // same F) instead of updating it again
+  bool reuse_response = false;
   FftPlan<float> fft32;
   float* spec32 = nullptr;
   int64_t evals32 = 0, evals64 = 0;
@@ -185,7 +204,7 @@ class Rve {
   hipGraphExec_t g_exec = nullptr;
   std::vector<char> g_sig, g_seen;
   int64_t stats_graph_captures = 0, stats_graph_updates = 0;
-  bool scal_pending = false, scal_rotate = false, scal_residual = false;
+  bool scal_pending = false, scal_rotate = false, scal_residual = false, scal_reused = false;
 
   // MechBase state (mech_base.h:75-107), column-major
   double C_ref[81], C_volAvg[81], C_minMaxAvg[81], C_volAvgLastInc[81], C_minMaxAvgLastInc[81], S[81];
@@ -197,17 +216,22 @@ class Rve {
   double err_BC = 0.0, err_div = 0.0;
   bool terminally_ill = false;
   int snes_iter = 0, nfuncs = 0, solve_evals = 0;
+  int solve_updates = 0;  // evaluations of this solve that updated the material point
   sc_stats stats;
   KTimer kt;
 
  private:
   std::vector<double*> allocs;
+  void setup(const sc_rve_desc& d, bool residual_graphs);
+  void release();
   double* dalloc(size_t n);
-  GammaHost gamma_host(bool do_gamma) const;
   int converged() const;
   int snes_ngmres();
   std::vector<double> reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
                              const std::vector<const double*>& b, int64_t n);
 };
+
+// host copy of a field of a context, sc_rve_get_field's layouts (capi.cpp)
+int rve_get_field(Rve& v, int field, double* out, int64_t n);
 
 }  // namespace sca

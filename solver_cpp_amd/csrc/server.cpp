@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "batch.h"
 #include "rve.h"
 
 namespace sca {
@@ -54,6 +55,7 @@ struct Job {
 struct Request {
   sc_request_result res{};
   bool done = false;
+  std::string err;  // message of a failed request (status 3), for the waiting client
 };
 
 }  // namespace
@@ -73,8 +75,12 @@ int64_t rve_footprint(const sc_rve_desc& d) {
 
 class Server {
  public:
-  Server(int64_t budget, int workers) : budget_(budget), t0_(clk::now()) {
+  // batch_wait_ms >= 0: the residual evaluations of the running jobs are
+  // computed in joint launches (Batcher), the oldest waiting at most that
+  // long for the others
+  Server(int64_t budget, int workers, double batch_wait_ms = -1.0) : budget_(budget), t0_(clk::now()) {
     if (workers < 1) workers = 1;
+    if (batch_wait_ms >= 0.0) batcher_.reset(new Batcher(batch_wait_ms));
     for (int w = 0; w < workers; ++w) threads_.emplace_back([this] { worker(); });
   }
   ~Server() {
@@ -84,9 +90,12 @@ class Server {
     }
     cv_work_.notify_all();
     for (auto& t : threads_) t.join();
+    jobs_.clear();  // contexts go before the batcher they point to
   }
 
   int64_t add_job(const sc_rve_desc& d, const sc_loadstep* steps, int n_steps, bool keep) {
+    validate_rve_desc(d);  // a job accepted here can be started
+    if (n_steps < 1 || !steps) throw std::runtime_error("a job needs at least one load step");
     auto j = std::make_unique<Job>();
     const int64_t N = (int64_t)d.cells[0] * d.cells[1] * d.cells[2];
     j->desc = d;
@@ -130,6 +139,7 @@ class Server {
     if (timeout_s < 0) cv_done_.wait(lk, done);
     else if (!cv_done_.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return 1;
     *out = reqs_[request].res;
+    if (!reqs_[request].err.empty()) g_err = reqs_[request].err;  // for sc_last_error() of the client
     reqs_.erase(request);
     return 0;
   }
@@ -152,6 +162,35 @@ class Server {
     cv_work_.notify_all();
   }
 
+  // a field of an idle resident job (the job is held busy meanwhile)
+  void get_field(int64_t job, int field, double* out, int64_t n) {
+    Job* j = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      auto it = jobs_.find(job);
+      if (it == jobs_.end()) throw std::runtime_error("unknown job");
+      j = it->second.get();
+      if (j->busy || !j->pending.empty()) throw std::runtime_error("job has a request running or pending");
+      if (!j->rve) throw std::runtime_error("job is not resident");
+      j->busy = true;
+    }
+    int rc = 1;
+    std::string err;
+    try {
+      rc = rve_get_field(*j->rve, field, out, n);
+    } catch (std::exception& e) {
+      err = e.what();
+    }
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      j->busy = false;
+      if (!j->pending.empty()) ready_.push_back(job);
+    }
+    cv_work_.notify_all();
+    if (!err.empty()) throw std::runtime_error(err);
+    if (rc != 0) throw std::runtime_error("field copy failed");
+  }
+
   void stats(sc_server_stats* o) {
     std::lock_guard<std::mutex> lk(m_);
     o->budget_bytes = budget_;
@@ -162,6 +201,12 @@ class Server {
     o->completed_requests = completed_;
     o->max_concurrent = max_concurrent_;
     o->evictions = evictions_;
+    if (batcher_) {
+      const BatchStats b = batcher_->stats();
+      o->batches = b.batches;
+      o->batched_evaluations = b.evaluations;
+      o->max_batch = b.max_batch;
+    }
   }
 
  private:
@@ -174,7 +219,7 @@ class Server {
     Job* v = nullptr;
     for (auto& kv : jobs_) {
       Job& j = *kv.second;
-      if (j.rve && !j.busy && j.pending.empty() && (!v || j.last_use < v->last_use)) v = &j;
+      if (j.rve && !j.busy && !j.failed && j.pending.empty() && (!v || j.last_use < v->last_use)) v = &j;
     }
     return v;
   }
@@ -183,7 +228,9 @@ class Server {
   bool pick(int64_t& job) {
     for (auto it = ready_.begin(); it != ready_.end(); ++it) {
       Job& j = *jobs_[*it];
-      if (j.rve || used_ + j.footprint <= budget_) {
+      // a failed job is never started again: its requests are answered
+      // without memory
+      if (j.rve || j.failed || used_ + j.footprint <= budget_) {
         job = *it;
         ready_.erase(it);
         return true;
@@ -212,7 +259,7 @@ class Server {
           j->busy = true;
           rid = j->pending.front();
           j->pending.pop_front();
-          if (!j->rve) {
+          if (!j->rve && !j->failed) {
             used_ += j->footprint;
             peak_ = std::max(peak_, used_);
             admit = true;
@@ -229,11 +276,13 @@ class Server {
       sc_request_result res{};
       res.request = rid;
       res.job = jid;
+      std::string err;
       try {
         if (admit) {
           // jobs evaluate eagerly: the per-job streams keep the device busy,
           // and short jobs would not amortise graph captures
           j->rve.reset(new Rve(j->desc, false));
+          j->rve->batch = batcher_.get();
           if (j->evicted) {
             j->rve->load(j->snapshot);
             j->snapshot.clear();
@@ -247,6 +296,7 @@ class Server {
           res.status = 2;
         } else {
           int iters = 0, evals = 0;
+          BatchMembership member(batcher_.get());
           const int rc = j->rve->next_increment(j->steps.data(), (int)j->steps.size(), j->cur, &iters, &evals);
           res.status = rc;
           if (rc == 2) j->failed = true;
@@ -260,17 +310,26 @@ class Server {
       } catch (std::exception& e) {
         res.status = 3;
         j->failed = true;
-        g_err = e.what();
+        err = e.what();
       }
       std::unique_ptr<Rve> drop;
       {
         std::lock_guard<std::mutex> lk(m_);
         --running_;
+        if (j->failed && j->rve) {
+          // a failed job holds no memory: its later requests are answered
+          // as failed without starting it again
+          used_ -= j->footprint;
+          drop = std::move(j->rve);
+        } else if (admit && !j->rve) {
+          used_ -= j->footprint;  // the start threw before the context existed
+        }
         Request& r = reqs_[rid];
         res.t_submit = r.res.t_submit;
         res.t_start = r.res.t_start;
         res.t_end = now();
         r.res = res;
+        r.err = err;
         r.done = true;
         ++completed_;
         j->busy = false;
@@ -279,7 +338,7 @@ class Server {
           ready_.push_front(jid);  // keep this job's increments flowing
         } else if (!j->keep_resident) {
           if (j->rve) used_ -= j->footprint;
-          drop = std::move(j->rve);
+          if (j->rve) drop = std::move(j->rve);
           jobs_.erase(jid);
         }
       }
@@ -306,12 +365,12 @@ class Server {
       if (ok) {
         v.snapshot = std::move(blob);
         v.evicted = true;
-        drop = std::move(v.rve);
-        used_ -= v.footprint;
         ++evictions_;
       } else {
-        v.failed = true;
+        v.failed = true;  // its state is lost: the job fails and gives its memory back
       }
+      drop = std::move(v.rve);
+      used_ -= v.footprint;
       v.busy = false;
       if (!v.pending.empty()) ready_.push_back(v.id);
     }
@@ -325,6 +384,7 @@ class Server {
   std::map<int64_t, Request> reqs_;
   std::deque<int64_t> ready_;
   std::vector<std::thread> threads_;
+  std::unique_ptr<Batcher> batcher_;
   int64_t budget_, used_ = 0, peak_ = 0, next_job_ = 0, next_req_ = 0, completed_ = 0, evictions_ = 0;
   uint64_t use_clock_ = 0;
   int running_ = 0, max_concurrent_ = 0;
@@ -348,6 +408,19 @@ int sc_server_create(int64_t hbm_budget_bytes, int workers, sc_server** out) {
   try {
     auto* s = new sc_server();
     s->s.reset(new sca::Server(hbm_budget_bytes, workers));
+    *out = s;
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int sc_server_create_batched(int64_t hbm_budget_bytes, int workers, double max_wait_ms, sc_server** out) {
+  try {
+    if (!(max_wait_ms >= 0.0)) throw std::runtime_error("max_wait_ms must be >= 0");
+    auto* s = new sc_server();
+    s->s.reset(new sca::Server(hbm_budget_bytes, workers, max_wait_ms));
     *out = s;
     return 0;
   } catch (std::exception& e) {
@@ -391,6 +464,16 @@ int sc_server_wait(sc_server* s, int64_t request, double timeout_s, sc_request_r
 int sc_server_remove_job(sc_server* s, int64_t job) {
   try {
     s->s->remove_job(job);
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+int sc_server_get_field(sc_server* s, int64_t job, int field, double* out, int64_t n_values) {
+  try {
+    s->s->get_field(job, field, out, n_values);
     return 0;
   } catch (std::exception& e) {
     g_err = e.what();

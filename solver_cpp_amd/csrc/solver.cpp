@@ -17,6 +17,7 @@
 #include <limits>
 #include <vector>
 
+#include "batch.h"
 #include "rve.h"
 
 namespace sca {
@@ -268,13 +269,48 @@ double* Rve::dalloc(size_t n) {
   return p;
 }
 
+// Every check a context's set-up makes, before anything is allocated: the
+// job server runs it when a job is added, so that a job it accepts can be
+// started.
+void validate_rve_desc(const sc_rve_desc& d) {
+  auto bad = [](const std::string& m) { throw std::runtime_error("invalid RVE description: " + m); };
+  if (d.cells[0] < 2) bad("cells[0] must be larger than 1");
+  for (int p = 0; p < 3; ++p) {
+    if (d.cells[p] < 1) bad("cells must be positive");
+    if (!(d.size[p] > 0.0)) bad("size must be positive");
+    int m = d.cells[p];
+    for (int f : {2, 3, 5, 7, 11, 13})
+      while (m % f == 0) m /= f;
+    if (m != 1) bad("grid length " + std::to_string(d.cells[p]) + " has a prime factor above 13");
+  }
+  const int64_t N = (int64_t)d.cells[0] * d.cells[1] * d.cells[2];
+  if (N > ((int64_t)1 << 29)) bad("more than 2^29 cells");
+  if (d.n_materials < 1) bad("no materials");
+  if (!d.material_at || !d.orientations) bad("null material map or orientations");
+  for (int64_t c = 0; c < N; ++c)
+    if (d.material_at[c] < 0 || d.material_at[c] >= d.n_materials) bad("material_at out of range");
+  if (d.precision != SC_PRECISION_FP64 && d.precision != SC_PRECISION_MIXED) bad("unknown precision");
+  if (d.grid_num.derivative < 0 || d.grid_num.derivative > 2) bad("unknown derivative");
+  if (d.grid_num.divergence_correction < 0 || d.grid_num.divergence_correction > 2)
+    bad("unknown divergence correction");
+}
+
 Rve::Rve(const sc_rve_desc& d, bool residual_graphs) {
+  validate_rve_desc(d);
+  try {
+    setup(d, residual_graphs);
+  } catch (...) {
+    release();  // a throwing constructor runs no destructor
+    throw;
+  }
+}
+
+void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
   desc = d;
   for (int p = 0; p < 3; ++p) {
     cells[p] = d.cells[p];
     size[p] = d.size[p];
   }
-  if (cells[0] < 2) throw std::runtime_error("cells[0] must be larger than 1");
   N = (int64_t)cells[0] * cells[1] * cells[2];
   nr = cells[0] / 2 + 1;
   K = (int64_t)nr * cells[1] * cells[2];
@@ -303,12 +339,14 @@ Rve::Rve(const sc_rve_desc& d, bool residual_graphs) {
   // each other through the null stream
   SCA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   fft.init(cells);
+  if (d.grid_num.fft_plan_mode == 1) fft.tune(9, st);
   ngm = new Ngmres(9 * N, st, &kt);
   ngm->after_sync = [this] { complete_response_scalars(); };
   SCA_HIP(hipHostMalloc((void**)&h_scal, 16 * sizeof(double), hipHostMallocDefault));
   mixed = d.precision == SC_PRECISION_MIXED;
   if (mixed) {
     fft32.init(cells);
+    if (d.grid_num.fft_plan_mode == 1) fft32.tune(9, st);
     SCA_HIP(hipMalloc(&spec32, 2 * 9 * K * sizeof(float)));
   }
   // material point planes
@@ -391,8 +429,7 @@ Rve::Rve(const sc_rve_desc& d, bool residual_graphs) {
     std::vector<double> om(9 * (size_t)d.n_materials);
     for (int m = 0; m < d.n_materials; ++m) qu2om_host(&orientations[4 * m], &om[9 * m]);
     for (int64_t c = 0; c < N; ++c) {
-      const int ma = material_at[c];
-      if (ma < 0 || ma >= d.n_materials) throw std::runtime_error("material_at out of range");
+      const int ma = material_at[c];  // in range: validate_rve_desc
       for (int q = 0; q < 9; ++q) fp[q * N + c] = om[9 * ma + q];
     }
     double* d_fp = dalloc(9 * N);
@@ -418,18 +455,29 @@ Rve::Rve(const sc_rve_desc& d, bool residual_graphs) {
   std::memset(&stats, 0, sizeof stats);
 }
 
-Rve::~Rve() {
+Rve::~Rve() { release(); }
+
+// frees whatever set-up allocated (also after a set-up that threw part way)
+void Rve::release() {
   release_residual_graph();
   delete ngm;
+  ngm = nullptr;
   if (h_scal) (void)hipHostFree(h_scal);
+  h_scal = nullptr;
   if (spec32) (void)hipFree(spec32);
+  spec32 = nullptr;
   fft32.release();
   for (double* p : allocs) (void)hipFree(p);
+  allocs.clear();
   if (mp.iters) (void)hipFree(mp.iters);
   if (mp.repivot) (void)hipFree(mp.repivot);
+  mp.iters = nullptr;
+  mp.repivot = nullptr;
   if (d_minmax_i) (void)hipFree(d_minmax_i);
+  d_minmax_i = nullptr;
   fft.release();
   if (st) (void)hipStreamDestroy(st);
+  st = nullptr;
 }
 
 // deterministic device reduction of nvec vectors of length n -> host values
@@ -488,6 +536,7 @@ void Rve::request_response_scalars(bool rotate, bool residual) {
   scal_pending = true;
   scal_rotate = rotate;
   scal_residual = residual;
+  scal_reused = false;
 }
 
 void Rve::complete_response_scalars() {
@@ -500,7 +549,7 @@ void Rve::complete_response_scalars() {
   if (scal_rotate) h::rot2(rot_q, P_av, P_av);
   last_div_sum = h_scal[10];
   if (ill) terminally_ill = true;
-  if (kt.on) {
+  if (kt.on && !scal_reused) {  // a reused update's iterations are counted once
     stats.newton_iterations += (int64_t)h_scal[11];
     stats.state_iterations += (int64_t)h_scal[12];
   }
@@ -596,6 +645,14 @@ void Rve::launch_residual(const double* Fdev, double* rdev) {
     launch_partial_reduce<int>(ops, N, 2, it, nullptr, d_scr_partial, st);
     launch_final_reduce(d_scr_partial, 2, red_blocks(), small + SM_ITERS, st);
   }
+  launch_spectral(rdev);
+}
+
+// The spectral part of an evaluation on the P of the last material-point
+// update: the aim of the convolution, forward transform, Gamma convolution
+// with the divergence, inverse transform into r, and the asynchronous copy of
+// the response scalars.
+void Rve::launch_spectral(double* rdev) {
   // DC of the convolution: rotated S : P_av, computed on the device
   double R[9], Rrm[9];
   qu2om_host(rot_q, R);
@@ -700,10 +757,23 @@ void Rve::release_residual_graph() {
 // MechSolverBasic::formResidual (mech_solver_basic.cpp:286-339).  Small grids
 // are bound by launch latency: there the evaluation is one graph launch
 // between copies of F in and r out (not while kernels are being timed).
+//
+// reuse_response (mixed precision, first evaluation of the FP64 finish): the
+// material point was last updated, in FP64, at exactly this F (the point the
+// FP32 stage returned was its last evaluation), so P and the material-point
+// state are those of F already; the evaluation runs the FP64 spectral stages
+// on them and counts no material-point update.
 void Rve::form_residual(const double* Fdev, double* rdev) {
   if (nfuncs == 0 && snes_iter == 0) total_iter = iter_base - 1;
   if (total_iter - iter_base <= snes_iter) total_iter += 1;
-  if (g_in && !kt.on && ensure_residual_graph()) {
+  const bool reused = reuse_response;
+  reuse_response = false;
+  if (reused) {
+    launch_spectral(rdev);
+  } else if (batch) {
+    SCA_HIP(hipStreamSynchronize(st));  // F and everything before it complete
+    batch->evaluate(this, Fdev, rdev);   // the response scalars land in h_scal
+  } else if (g_in && !kt.on && ensure_residual_graph()) {
     SCA_HIP(hipMemcpyAsync(g_in, Fdev, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
     SCA_HIP(hipGraphLaunch(g_exec, st));
     SCA_HIP(hipMemcpyAsync(rdev, g_out, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -717,6 +787,7 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
   scal_pending = true;
   scal_rotate = true;
   scal_residual = true;
+  scal_reused = reused;
   if (use32) {
     evals32++;
     stats.fp32_evaluations++;
@@ -726,15 +797,22 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
   nfuncs++;
   solve_evals++;
   stats.residual_evaluations++;
-  stats.material_point_updates += N;
+  if (!reused) {
+    stats.material_point_updates += N;
+    solve_updates++;
+  }
 }
 
 // MechSolverBasic::converged, mech_solver_basic.cpp:242-284
+// The FP32 stage of the mixed form stops at a divergence tolerance no
+// tighter than MIXED_FP32_REL_DIV_MIN (its spectra carry FP32 rounding); the
+// FP64 solve that follows holds the configured one.
 int Rve::converged() const {
   const auto& gn = desc.grid_num;
   double maxP = -std::numeric_limits<double>::infinity();
   for (int q = 0; q < 9; ++q) maxP = std::max(maxP, P_av[q]);
-  const double divTol = std::max(maxP * gn.eps_rel_div, gn.eps_abs_div);
+  const double eps_div = use32 ? std::max(gn.eps_rel_div, MIXED_FP32_REL_DIV_MIN) : gn.eps_rel_div;
+  const double divTol = std::max(maxP * eps_div, gn.eps_abs_div);
   const double BCTol = std::max(maxP * gn.eps_rel_P, gn.eps_abs_P);
   if ((total_iter >= gn.itmin && std::max(err_div / divTol, err_BC / BCTol) < 1.0) || terminally_ill) return 1;
   if (total_iter >= gn.itmax) return -1;
@@ -815,7 +893,14 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
   Xh.assign(msize, nullptr);
   Xc = X;
   // the solution into the caller's X; every reference released
+  // every evaluation goes through here: the point of the last one is kept
+  const double* last_x = nullptr;
+  auto eval = [&](const double* x, double* f) {
+    last_x = x;
+    residual(x, f);
+  };
   auto done = [&](int r) {
+    solution_evaluated_last = Xc == last_x;
     if (Xc != X) SCA_HIP(hipMemcpyAsync(X, Xc, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     Fv = Xc = XM = FM = XA = FA = nullptr;
     Fh.assign(msize, nullptr);
@@ -824,7 +909,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
   };
   Fv = take();
   set_iter(0);
-  residual(Xc, Fv);
+  eval(Xc, Fv);
   double fnorm = std::sqrt(reduce({0}, {Fv}, {Fv})[0]);
   if (std::isnan(fnorm) || std::isinf(fnorm)) return done(-4);  // SNES_DIVERGED_FNORM_NAN
   double fminnorm = fnorm;
@@ -846,7 +931,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
     if (kt) kt->end(SC_KT_REDUCE, n, st);
     FM = nullptr;
     FM = take();
-    residual(XM, FM);
+    eval(XM, FM);
     // ||FM||^2, <FM, Fdot_i>, <Fdot_ivec, Fdot_i>: one pass over FM,
     // Fdot_ivec and the history (each vector read once)
     std::vector<double> dots(1 + 2 * l);
@@ -910,7 +995,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       if (kt) kt->end(SC_KT_REDUCE, n * (l + 1), st);
       FA = nullptr;
       FA = take();
-      residual(XA, FA);
+      eval(XA, FA);
     }
     if (fminnorm > fMnorm) fminnorm = fMnorm;
     // ||FA||^2, ||XA - XM||^2, ||Xdot_i - XA||^2 (SNESNGMRESNorms_Private):
@@ -1014,10 +1099,46 @@ void Rve::update_coords() {
   launch_coords_points(d_u, d_fsum, cells, size, d_xp, st);
 }
 
-// Checkpoint layout: a header (magic, version, N, cells, precision), the
-// scalar state of the solver, then the device planes in a fixed order.
+// Checkpoint layout: a header (magic, format version, N, cells, precision
+// and a hash of the description's material map, orientations, material and
+// numerics), the scalar state of the solver, then the device planes in a
+// fixed order.
 namespace {
 constexpr char CKPT_MAGIC[8] = {'S', 'C', 'A', 'M', 'D', 'C', 'K', '1'};
+constexpr int32_t CKPT_VERSION = 2;
+
+// FNV-1a over the fields of a description that a checkpoint's state depends
+// on (field by field: the structs have padding)
+uint64_t desc_hash(const sc_rve_desc& d, int64_t N) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  };
+  auto f64 = [&](double v) { mix(&v, sizeof v); };
+  auto i32 = [&](int v) { mix(&v, sizeof v); };
+  mix(d.material_at, (size_t)N * sizeof(int));
+  i32(d.n_materials);
+  mix(d.orientations, (size_t)4 * d.n_materials * sizeof(double));
+  const sc_material& m = d.material;
+  for (double v : {m.C_11, m.C_12, m.C_44, m.xi_0_sl, m.xi_inf_sl, m.h_0_sl_sl, m.n_sl, m.a_sl, m.dot_gamma_0_sl,
+                   m.atol_xi, m.atol_gamma})
+    f64(v);
+  for (double v : m.h_sl_sl) f64(v);
+  const sc_phase_numerics& p = d.phase_num;
+  for (double v : {p.r_cutback_min, p.r_cutback, p.r_increase, p.eps_rel_state, p.r_linesearch_Lp, p.eps_rel_Lp,
+                   p.eps_abs_Lp})
+    f64(v);
+  i32(p.N_iter_state_max);
+  i32(p.N_iter_Lp_max);
+  const sc_grid_numerics& g = d.grid_num;
+  for (int v : {g.itmin, g.itmax, g.update_gamma, g.divergence_correction, g.derivative, g.stag_iter_max,
+                g.max_cut_back, g.ngmres_m})
+    i32(v);
+  for (double v : {g.eps_abs_div, g.eps_rel_div, g.eps_abs_P, g.eps_rel_P}) f64(v);
+  for (int q = 0; q < 3; ++q) f64(d.size[q]);
+  return h;
+}
 struct CkptScalars {
   double C_ref[81], C_volAvg[81], C_minMaxAvg[81], C_volAvgLastInc[81], C_minMaxAvgLastInc[81], S[81];
   double P_av[9], P_aim[9], F_aim_dot[9], F_aim[9], F_aim_last_inc[9];
@@ -1030,9 +1151,12 @@ struct CkptScalars {
 };
 struct CkptHeader {
   char magic[8];
+  int32_t version;
+  int32_t precision;
   int64_t N;
-  int cells[3];
-  int precision;
+  int32_t cells[3];
+  int32_t pad;
+  uint64_t desc_hash;
 };
 }  // namespace
 
@@ -1049,9 +1173,11 @@ void Rve::save(std::vector<char>& blob) {
   for (auto& pl : planes) doubles += pl.second;
   CkptHeader h{};
   std::memcpy(h.magic, CKPT_MAGIC, 8);
+  h.version = CKPT_VERSION;
   h.N = N;
   for (int p = 0; p < 3; ++p) h.cells[p] = cells[p];
   h.precision = desc.precision;
+  h.desc_hash = desc_hash(desc, N);
   CkptScalars sc{};
   std::memcpy(sc.C_ref, C_ref, sizeof C_ref);
   std::memcpy(sc.C_volAvg, C_volAvg, sizeof C_volAvg);
@@ -1099,9 +1225,16 @@ void Rve::load(const std::vector<char>& blob) {
   if (blob.size() < sizeof h + sizeof sc) throw std::runtime_error("checkpoint too short");
   std::memcpy(&h, blob.data(), sizeof h);
   if (std::memcmp(h.magic, CKPT_MAGIC, 8) != 0) throw std::runtime_error("not a solver checkpoint");
+  if (h.version != CKPT_VERSION) throw std::runtime_error("checkpoint of another format version");
   if (h.N != N || h.cells[0] != cells[0] || h.cells[1] != cells[1] || h.cells[2] != cells[2] ||
       h.precision != desc.precision)
     throw std::runtime_error("checkpoint of another grid or precision");
+  if (h.desc_hash != desc_hash(desc, N))
+    throw std::runtime_error("checkpoint of another microstructure, material or numerics");
+  // nothing of this context's last evaluation may land on the restored state
+  complete_response_scalars();
+  release_residual_graph();
+  g_seen.clear();
   const std::vector<std::pair<double*, int64_t>> planes = {
       {mp.F, 9 * N},       {mp.F0, 9 * N},    {mp.Fp0, 9 * N},   {mp.Lp0, 9 * N},   {mp.S0, 9 * N},
       {mp.xi0, 12 * N},    {mp.gamma0, 12 * N}, {mp.Lp, 9 * N},  {mp.P, 9 * N},     {mp.Fp, 9 * N},
@@ -1147,6 +1280,7 @@ void Rve::load(const std::vector<char>& blob) {
     off += pl.second * sizeof(double);
   }
   f_eval = nullptr;  // mp.F holds the restored F
+  scal_pending = false;
   SCA_HIP(hipStreamSynchronize(st));
 }
 
@@ -1218,19 +1352,27 @@ sc_solution Rve::calculate_solution() {
   if (desc.grid_num.update_gamma)
     for (int q = 0; q < 81; ++q) C_ref[q] = C_minMaxAvg[q] / wgt;
   solve_evals = 0;
+  solve_updates = 0;
   iter_base = 0;
   int reason;
   if (mixed) {
-    // FP32 spectral stages until the FP64-normed criteria hold, then an FP64
-    // solve continued from that F (its iterations count on from the first
-    // solve's), which accepts at its first evaluation when the FP64 residual
-    // already meets the tolerances
+    // FP32 spectral stages until the criteria hold (the divergence at no
+    // tighter than MIXED_FP32_REL_DIV_MIN), then an FP64 solve continued from
+    // that F (its iterations count on from the first solve's), which accepts
+    // at its first evaluation when the FP64 residual already meets the
+    // tolerances.  The FP64 solve also takes over when the FP32 stage stopped
+    // without converging (iteration limit, a non-finite norm): only a
+    // terminally ill point ends the increment there.  When the FP32 stage's
+    // last evaluation was at the F it returned, the FP64 solve's first
+    // evaluation reuses that FP64 material-point update.
     use32 = true;
     reason = snes_ngmres();
     use32 = false;
-    if (reason > 0 && !terminally_ill) {
+    if (!terminally_ill) {
       iter_base = total_iter + 1;
+      reuse_response = ngm->solution_evaluated_last;
       reason = snes_ngmres();
+      reuse_response = false;
     }
     iter_base = 0;
   } else {

@@ -11,13 +11,21 @@ constexpr int MAX_RED_VEC = 96;
 
 // 3D FFT of `ncomp` real planes (N each) <-> half spectra (K complex each)
 template <typename T>
+struct cpx;
+template <typename T>
 struct FftPlan {
   int cells[3] = {0, 0, 0};
   int nr = 0;
   int64_t N = 0, K = 0;
   T* tw[3] = {nullptr, nullptr, nullptr};  // forward twiddles per axis
+  // launch shape per axis pass (pencil block size, or pencils per block of
+  // the generic kernels); 0: the heuristic one.  No shape changes results.
+  int cfg[3] = {0, 0, 0};
   void init(const int cells[3]);
   void release();
+  // times every launch shape for transforms of ncomp components and keeps
+  // the fastest (measured once per precision, grid and ncomp per process)
+  void tune(int ncomp, hipStream_t st);
   // real: ncomp planes of N (double or T, converted in the x pass); spec:
   // ncomp planes of K complex T (interleaved re, im)
   template <typename TI>
@@ -25,6 +33,11 @@ struct FftPlan {
   template <typename TO>
   void backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st) const;  // spec is clobbered
   void strided_passes(void* spec, int ncomp, int sign, hipStream_t st) const;
+
+ private:
+  int shape(int axis) const;
+  void x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp, hipStream_t st) const;
+  void strided_pass(cpx<T>* S, int ncomp, int sign, int axis, int shp, hipStream_t st) const;
 };
 
 struct GammaHost {
@@ -88,5 +101,42 @@ void launch_soa_to_aos(const double* soa, double* aos, int64_t N, int ncomp, hip
 void launch_aos_to_soa(const double* aos, double* soa, int64_t N, int ncomp, hipStream_t st);
 void launch_f64_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
 void launch_f32_to_f64(const float* a, double* b, int64_t n, hipStream_t st);
+
+// ---- launches over a batch of RVEs (tables in a TableArena, common.h) -----
+// out[v] = sum_b partial[v*nb + b] for v < nvec, per entry (final_reduce)
+struct BatchReduce {
+  const double* partial;
+  int nb, nvec;
+  double* out;
+};
+void launch_final_reduce_batch(const BatchReduce* items, int n, TableArena& ta, hipStream_t st);
+// dst[i] = src[i] for i < n, per entry
+struct BatchCopy {
+  const double* src;
+  double* dst;
+  int64_t n;
+};
+void launch_copy_batch(const BatchCopy* items, int n, TableArena& ta, hipStream_t st);
+// the DC aim S : (R P_av R^T) of each RVE (launch_bc_aim)
+struct BcBatchItem {
+  const double* sums;
+  double S[81], R[9], wgt;
+  int rotate;
+  double* aim;
+};
+void launch_bc_aim_batch(const BcBatchItem* items, int n, TableArena& ta, hipStream_t st);
+// Gamma convolution with the divergence of RVEs with the same grid (K
+// wavevectors): spec of each (9 planes of K complex T), as launch_gamma_div
+struct GammaBatchItem {
+  void* spec;
+  GammaHost gh;
+  const double* d_aim;
+  const double* d_xi_tab;
+  double* div_partial;
+};
+template <typename T>
+void launch_gamma_div_batch(const GammaBatchItem* items, int n, int64_t K, TableArena& ta, hipStream_t st);
+// bytes of table space one RVE of a batch needs at most, all launches together
+size_t batch_table_bytes_per_rve();
 
 }  // namespace sca

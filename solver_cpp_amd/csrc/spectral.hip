@@ -10,6 +10,11 @@
 // unnormalised with e^{-2 pi i k.x/N} (FFTW r2c, src/grid/fft.cpp:48-52),
 // inverse transforms e^{+2 pi i ...} scaled by a caller-given factor
 // (src/grid/fft.h:50-60).
+#include <array>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "spectral.h"
 
 namespace sca {
@@ -243,6 +248,13 @@ struct GammaArgs {
   const double* aim;     // 9 doubles (column-major): DC value
   int do_gamma;
 };
+// one RVE of a batched Gamma convolution: its spectrum (9 planes of K
+// complex), arguments and divergence partial sums
+struct GammaBatchEntry {
+  void* data;
+  GammaArgs g;
+  double* div_partial;
+};
 
 // xi of wavevector (i, y, z): continuous derivative on the fly (xi1st zeroes
 // each component at the Nyquist index of an even dimension, xi2nd does not),
@@ -271,9 +283,9 @@ __device__ __forceinline__ void xi_at(const GammaArgs& g, int i, int y, int z, c
 // partial sums, fixed tree) and P_hat <- Gamma_hat : P_hat (or the aim at
 // k = 0).  data: 9 complex planes of K.
 template <typename T>
-__global__ void __launch_bounds__(256) gamma_div_kernel(cpx<T>* data, int64_t K, GammaArgs g, double* div_partial) {
-  __shared__ double red[256];
-  const int64_t kk = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void gamma_div_block(cpx<T>* data, int64_t K, const GammaArgs& g, double* div_partial,
+                                                unsigned blk, double* red) {
+  const int64_t kk = (int64_t)blk * 256 + threadIdx.x;
   double contrib = 0.0;
   if (kk < K) {
     const int i = (int)(kk % g.nr);
@@ -387,8 +399,30 @@ __global__ void __launch_bounds__(256) gamma_div_kernel(cpx<T>* data, int64_t K,
       if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
       __syncthreads();
     }
-    if (threadIdx.x == 0) div_partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) div_partial[blk] = red[0];
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) gamma_div_kernel(cpx<T>* data, int64_t K, GammaArgs g, double* div_partial) {
+  __shared__ double red[256];
+  gamma_div_block<T>(data, K, g, div_partial, blockIdx.x, red);
+}
+
+// A batch of RVEs with the same grid: blockIdx.y is the RVE, each block does
+// what its own launch does (same wavevectors, same partial sums).  The table
+// is read through the constant address space (invariant loads, as for kernel
+// arguments).
+template <typename T>
+__global__ void __launch_bounds__(256) gamma_div_batch_kernel(const GammaBatchEntry* __restrict__ tab, int64_t K) {
+  __shared__ double red[256];
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(4))) const GammaBatchEntry ConstEntry;
+  const GammaBatchEntry e = *(const ConstEntry*)(tab + blockIdx.y);
+#else
+  const GammaBatchEntry e = tab[blockIdx.y];
+#endif
+  gamma_div_block<T>(reinterpret_cast<cpx<T>*>(e.data), K, e.g, e.div_partial, blockIdx.x, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -647,6 +681,31 @@ __global__ void __launch_bounds__(256) final_reduce_kernel(const double* partial
   if (threadIdx.x == 0) out[v] = red[0];
 }
 
+// final reductions of a batch: blockIdx.y is the entry, blockIdx.x the
+// vector, each as final_reduce_kernel (same tree)
+__global__ void __launch_bounds__(256) final_reduce_batch_kernel(const BatchReduce* __restrict__ tab) {
+  __shared__ double red[256];
+  const BatchReduce e = tab[blockIdx.y];
+  const int v = blockIdx.x;
+  if (v >= e.nvec) return;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < e.nb; t += 256) s += e.partial[(int64_t)v * e.nb + t];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) e.out[v] = red[0];
+}
+
+// copies of a batch (gathers and scatters between RVE buffers and a group's
+// contiguous arena): blockIdx.y is the entry
+__global__ void __launch_bounds__(256) copy_batch_kernel(const BatchCopy* __restrict__ tab) {
+  const BatchCopy e = tab[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < e.n; i += (int64_t)gridDim.x * 256) e.dst[i] = e.src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Streaming field ops
 // ---------------------------------------------------------------------------
@@ -806,23 +865,59 @@ static FftPassDesc make_desc(int n, int sign) {
 }
 
 // Pencil kernels for the supported lengths (compile-time N); false: use the
-// generic Stockham kernels
+// generic Stockham kernels.  bs: threads per block (pencils per block =
+// bs / threads per pencil); any valid value gives the same results.
 #define SCA_PENCIL_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(12) X(24) X(48) X(96) X(192) X(384)
 
+constexpr size_t MAX_BLOCK_LDS = 64 * 1024;
+
 template <typename T, int N>
-static size_t pencil_lds() {
-  return (size_t)pencil::Shape<T, N>::LDS * sizeof(cpx<T>);
+static size_t pencil_lds(int bs) {
+  return (size_t)pencil::Shape<T, N>::lds(bs) * sizeof(cpx<T>);
+}
+
+// threads per pencil of a pencil length (0: not a pencil length)
+template <typename T>
+static int pencil_tp(int n) {
+  switch (n) {
+#define SCA_CASE(N) \
+  case N: return pencil::Shape<T, N>::TP;
+    SCA_PENCIL_LENGTHS(SCA_CASE)
+#undef SCA_CASE
+    default: return 0;
+  }
+}
+template <typename T>
+static size_t pencil_lds_of(int n, int bs) {
+  switch (n) {
+#define SCA_CASE(N) \
+  case N: return pencil_lds<T, N>(bs);
+    SCA_PENCIL_LENGTHS(SCA_CASE)
+#undef SCA_CASE
+    default: return 0;
+  }
+}
+// block sizes a pencil pass may take: powers of two from the pencil's
+// threads (and 64) to 512 within the LDS limit
+template <typename T>
+static std::vector<int> pencil_block_sizes(int n) {
+  std::vector<int> out;
+  const int tp = pencil_tp<T>(n);
+  if (tp == 0) return out;
+  for (int bs = 64; bs <= pencil::MAX_PENCIL_BLOCK; bs *= 2)
+    if (bs >= tp && pencil_lds_of<T>(n, bs) <= MAX_BLOCK_LDS) out.push_back(bs);
+  return out;
 }
 
 template <typename T, typename TI>
-static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, const cpx<T>* tw, hipStream_t st) {
+static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, const cpx<T>* tw, int bs, hipStream_t st) {
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
     using S = pencil::Shape<T, N>;                                                                              \
-    const int64_t pencils = (rows + 1) / 2;                                                                  \
-    hipLaunchKernelGGL((pencil::x_r2c_kernel<T, TI, N>), dim3((unsigned)((pencils + S::P - 1) / S::P)),     \
-                       dim3(256), (pencil_lds<T, N>()), st, in, out, rows, tw);                                \
+    const int64_t pencils = (rows + 1) / 2, P = bs / S::TP;                                                  \
+    hipLaunchKernelGGL((pencil::x_r2c_kernel<T, TI, N>), dim3((unsigned)((pencils + P - 1) / P)), dim3(bs),   \
+                       (pencil_lds<T, N>(bs)), st, in, out, rows, tw);                                       \
     return true;                                                                                             \
   }
     SCA_PENCIL_LENGTHS(SCA_CASE)
@@ -832,15 +927,15 @@ static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, const c
 }
 
 template <typename T, typename TO>
-static bool pencil_x_c2r(int n, const cpx<T>* in, TO* out, int64_t rows, const cpx<T>* tw, T scale,
+static bool pencil_x_c2r(int n, const cpx<T>* in, TO* out, int64_t rows, const cpx<T>* tw, T scale, int bs,
                          hipStream_t st) {
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
     using S = pencil::Shape<T, N>;                                                                              \
-    const int64_t pencils = (rows + 1) / 2;                                                                  \
-    hipLaunchKernelGGL((pencil::x_c2r_kernel<T, TO, N>), dim3((unsigned)((pencils + S::P - 1) / S::P)),     \
-                       dim3(256), (pencil_lds<T, N>()), st, in, out, rows, tw, scale);                         \
+    const int64_t pencils = (rows + 1) / 2, P = bs / S::TP;                                                  \
+    hipLaunchKernelGGL((pencil::x_c2r_kernel<T, TO, N>), dim3((unsigned)((pencils + P - 1) / P)), dim3(bs),   \
+                       (pencil_lds<T, N>(bs)), st, in, out, rows, tw, scale);                                \
     return true;                                                                                             \
   }
     SCA_PENCIL_LENGTHS(SCA_CASE)
@@ -851,17 +946,18 @@ static bool pencil_x_c2r(int n, const cpx<T>* in, TO* out, int64_t rows, const c
 
 template <typename T>
 static bool pencil_strided(int n, int sign, cpx<T>* data, int ni, unsigned no, int d1, int64_t s1, int64_t s2,
-                           int64_t es, const cpx<T>* tw, hipStream_t st) {
+                           int64_t es, const cpx<T>* tw, int bs, hipStream_t st) {
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
     using S = pencil::Shape<T, N>;                                                                              \
-    const dim3 grid((unsigned)((ni + S::P - 1) / S::P), no);                                                \
+    const int P = bs / S::TP;                                                                                \
+    const dim3 grid((unsigned)((ni + P - 1) / P), no);                                                      \
     if (sign < 0)                                                                                            \
-      hipLaunchKernelGGL((pencil::strided_kernel<T, N, -1>), grid, dim3(256), (pencil_lds<T, N>()), st, data, \
+      hipLaunchKernelGGL((pencil::strided_kernel<T, N, -1>), grid, dim3(bs), (pencil_lds<T, N>(bs)), st, data, \
                          ni, d1, s1, s2, es, tw);                                                            \
     else                                                                                                     \
-      hipLaunchKernelGGL((pencil::strided_kernel<T, N, 1>), grid, dim3(256), (pencil_lds<T, N>()), st, data,  \
+      hipLaunchKernelGGL((pencil::strided_kernel<T, N, 1>), grid, dim3(bs), (pencil_lds<T, N>(bs)), st, data,  \
                          ni, d1, s1, s2, es, tw);                                                            \
     return true;                                                                                             \
   }
@@ -869,6 +965,24 @@ static bool pencil_strided(int n, int sign, cpx<T>* data, int ni, unsigned no, i
 #undef SCA_CASE
     default: return false;
   }
+}
+
+template <typename T>
+static int pencils_per_block(int n) {
+  int PB = 16;
+  while (PB > 1 && (size_t)2 * PB * (n + 1) * sizeof(cpx<T>) > MAX_BLOCK_LDS) PB /= 2;
+  return PB;
+}
+static int x_rows_per_block(int n) { return std::max(1, std::min(256, 2048 / n)); }
+
+// launch shape of an axis pass: the pencil block size for pencil lengths, the
+// pencils (rows) per block of the generic kernels otherwise; 0 = heuristic
+template <typename T>
+int FftPlan<T>::shape(int axis) const {
+  if (cfg[axis] > 0) return cfg[axis];
+  const int n = cells[axis];
+  if (pencil_tp<T>(n) > 0) return 256;
+  return axis == 0 ? x_rows_per_block(n) : pencils_per_block<T>(n);
 }
 
 template <typename T>
@@ -901,59 +1015,59 @@ void FftPlan<T>::release() {
 }
 
 template <typename T>
-static int pencils_per_block(int n) {
-  int PB = 16;
-  while (PB > 1 && (size_t)2 * PB * (n + 1) * sizeof(cpx<T>) > 64 * 1024) PB /= 2;
-  return PB;
+void FftPlan<T>::x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp,
+                            hipStream_t st) const {
+  const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
+  const int64_t rows = (int64_t)ncomp * Ny * Nz;
+  const cpx<T>* t0 = reinterpret_cast<const cpx<T>*>(tw[0]);
+  bool done;
+  if (real_is_double) done = pencil_x_r2c<T, double>(Nx, static_cast<const double*>(real), S, rows, t0, shp, st);
+  else done = pencil_x_r2c<T, T>(Nx, static_cast<const T*>(real), S, rows, t0, shp, st);
+  if (!done) {
+    FftPassDesc d = make_desc(Nx, -1);
+    const size_t sh = 2 * (size_t)shp * Nx * sizeof(cpx<T>);
+    if (real_is_double)
+      hipLaunchKernelGGL((fft_x_r2c<T, double>), dim3(nblk(rows, shp)), dim3(256), sh, st,
+                         static_cast<const double*>(real), S, rows, d, t0, shp);
+    else
+      hipLaunchKernelGGL((fft_x_r2c<T, T>), dim3(nblk(rows, shp)), dim3(256), sh, st, static_cast<const T*>(real), S,
+                         rows, d, t0, shp);
+  }
+  SCA_HIP(hipGetLastError());
 }
-static int x_rows_per_block(int n) { return std::max(1, std::min(256, 2048 / n)); }
 
 template <typename T>
 template <typename TI>
 void FftPlan<T>::forward(const TI* real, T* spec, int ncomp, hipStream_t st) const {
-  const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
-  // x: rows of all components
-  if (pencil_x_r2c<T, TI>(Nx, real, S, (int64_t)ncomp * Ny * Nz, reinterpret_cast<const cpx<T>*>(tw[0]), st)) {
-    SCA_HIP(hipGetLastError());
-  } else {
-    FftPassDesc d = make_desc(Nx, -1);
-    const int PB = x_rows_per_block(Nx);
-    const int64_t rows = (int64_t)ncomp * Ny * Nz;
-    const size_t sh = 2 * (size_t)PB * Nx * sizeof(cpx<T>);
-    hipLaunchKernelGGL((fft_x_r2c<T, TI>), dim3(nblk(rows, PB)), dim3(256), sh, st, real, S, rows, d,
-                       reinterpret_cast<const cpx<T>*>(tw[0]), PB);
-    SCA_HIP(hipGetLastError());
-  }
+  x_pass_fwd(real, sizeof(TI) == sizeof(double), S, ncomp, shape(0), st);
   strided_passes(S, ncomp, -1, st);
+}
+
+template <typename T>
+void FftPlan<T>::strided_pass(cpx<T>* S, int ncomp, int sign, int axis, int shp, hipStream_t st) const {
+  const int Ny = cells[1], Nz = cells[2];
+  const int n = cells[axis];
+  // y: pencils (c, z, i) with element stride nr; z: pencils (c, y, i), stride nr*Ny
+  const unsigned no = (unsigned)(ncomp * (axis == 1 ? Nz : Ny));
+  const int d1 = axis == 1 ? Nz : Ny;
+  const int64_t s2 = axis == 1 ? (int64_t)nr * Ny : (int64_t)nr;
+  const int64_t es = axis == 1 ? (int64_t)nr : (int64_t)nr * Ny;
+  const cpx<T>* t = reinterpret_cast<const cpx<T>*>(tw[axis]);
+  if (!pencil_strided<T>(n, sign, S, nr, no, d1, K, s2, es, t, shp, st)) {
+    FftPassDesc d = make_desc(n, sign);
+    dim3 grid((nr + shp - 1) / shp, no);
+    hipLaunchKernelGGL(fft_strided<T>, grid, dim3(256), 2 * (size_t)shp * (n + 1) * sizeof(cpx<T>), st, S, nr, d1, K,
+                       s2, es, d, t, shp);
+  }
+  SCA_HIP(hipGetLastError());
 }
 
 template <typename T>
 void FftPlan<T>::strided_passes(void* Sv, int ncomp, int sign, hipStream_t st) const {
   cpx<T>* S = reinterpret_cast<cpx<T>*>(Sv);
-  const int Ny = cells[1], Nz = cells[2];
-  if (Ny > 1 && pencil_strided<T>(Ny, sign, S, nr, (unsigned)(ncomp * Nz), Nz, K, (int64_t)nr * Ny, (int64_t)nr,
-                                   reinterpret_cast<const cpx<T>*>(tw[1]), st)) {
-    SCA_HIP(hipGetLastError());
-  } else if (Ny > 1) {  // y: pencils (c, z, i), stride nr
-    FftPassDesc d = make_desc(Ny, sign);
-    const int PB = pencils_per_block<T>(Ny);
-    dim3 grid((nr + PB - 1) / PB, (unsigned)(ncomp * Nz));
-    hipLaunchKernelGGL(fft_strided<T>, grid, dim3(256), 2 * (size_t)PB * (Ny + 1) * sizeof(cpx<T>), st, S, nr, Nz, K,
-                       (int64_t)nr * Ny, (int64_t)nr, d, reinterpret_cast<const cpx<T>*>(tw[1]), PB);
-    SCA_HIP(hipGetLastError());
-  }
-  if (Nz > 1 && pencil_strided<T>(Nz, sign, S, nr, (unsigned)(ncomp * Ny), Ny, K, (int64_t)nr, (int64_t)nr * Ny,
-                                   reinterpret_cast<const cpx<T>*>(tw[2]), st)) {
-    SCA_HIP(hipGetLastError());
-  } else if (Nz > 1) {  // z: pencils (c, y, i), stride nr*Ny
-    FftPassDesc d = make_desc(Nz, sign);
-    const int PB = pencils_per_block<T>(Nz);
-    dim3 grid((nr + PB - 1) / PB, (unsigned)(ncomp * Ny));
-    hipLaunchKernelGGL(fft_strided<T>, grid, dim3(256), 2 * (size_t)PB * (Nz + 1) * sizeof(cpx<T>), st, S, nr, Ny, K,
-                       (int64_t)nr, (int64_t)nr * Ny, d, reinterpret_cast<const cpx<T>*>(tw[2]), PB);
-    SCA_HIP(hipGetLastError());
-  }
+  if (cells[1] > 1) strided_pass(S, ncomp, sign, 1, shape(1), st);
+  if (cells[2] > 1) strided_pass(S, ncomp, sign, 2, shape(2), st);
 }
 
 template <typename T>
@@ -962,18 +1076,102 @@ void FftPlan<T>::backward(T* spec, TO* real, int ncomp, double scale, hipStream_
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
   strided_passes(S, ncomp, +1, st);
+  const int shp = shape(0);
   if (pencil_x_c2r<T, TO>(Nx, S, real, (int64_t)ncomp * Ny * Nz, reinterpret_cast<const cpx<T>*>(tw[0]), (T)scale,
-                          st)) {
+                          shp, st)) {
     SCA_HIP(hipGetLastError());
     return;
   }
   FftPassDesc d = make_desc(Nx, +1);
-  const int PB = x_rows_per_block(Nx);
   const int64_t rows = (int64_t)ncomp * Ny * Nz;
-  const size_t sh = 2 * (size_t)PB * Nx * sizeof(cpx<T>);
-  hipLaunchKernelGGL((fft_x_c2r<T, TO>), dim3(nblk(rows, PB)), dim3(256), sh, st, S, real, rows, d,
-                     reinterpret_cast<const cpx<T>*>(tw[0]), (T)scale, PB);
+  const size_t sh = 2 * (size_t)shp * Nx * sizeof(cpx<T>);
+  hipLaunchKernelGGL((fft_x_c2r<T, TO>), dim3(nblk(rows, shp)), dim3(256), sh, st, S, real, rows, d,
+                     reinterpret_cast<const cpx<T>*>(tw[0]), (T)scale, shp);
   SCA_HIP(hipGetLastError());
+}
+
+// Autotuner: for every axis pass, each launch shape (pencil block sizes, or
+// the generic kernels' pencils per block within the LDS limit) is timed on
+// scratch arrays of the plan's size for ncomp components, and the fastest
+// kept.  Shapes change no arithmetic, so a tuned plan computes bitwise what
+// the heuristic one does.  Results are kept per (precision, grid, ncomp) for
+// the process: later plans of the same kind take them without timing.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, std::array<int, 3>> g_tuned;
+}  // namespace
+
+template <typename T>
+void FftPlan<T>::tune(int ncomp, hipStream_t st) {
+  const auto key = std::make_tuple((int)sizeof(T), cells[0], cells[1], cells[2], ncomp);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      for (int a = 0; a < 3; ++a) cfg[a] = it->second[a];
+      return;
+    }
+  }
+  double* real = nullptr;
+  T* spec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::array<int, 3> best = {0, 0, 0};
+  try {
+    SCA_HIP(hipMalloc(&real, (size_t)ncomp * N * sizeof(double)));
+    SCA_HIP(hipMalloc(&spec, (size_t)2 * ncomp * K * sizeof(T)));
+    SCA_HIP(hipMemsetAsync(real, 0, (size_t)ncomp * N * sizeof(double), st));
+    SCA_HIP(hipMemsetAsync(spec, 0, (size_t)2 * ncomp * K * sizeof(T), st));
+    SCA_HIP(hipEventCreate(&e0));
+    SCA_HIP(hipEventCreate(&e1));
+    cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
+    for (int axis = 0; axis < 3; ++axis) {
+      const int n = cells[axis];
+      if (axis > 0 && n <= 1) continue;
+      std::vector<int> cands = pencil_block_sizes<T>(n);
+      if (cands.empty()) {  // generic kernels: pencils (rows) per block
+        const int lim = axis == 0 ? 256 : 64;
+        for (int pb = 1; pb <= lim; pb *= 2) {
+          const size_t lds = 2 * (size_t)pb * (n + (axis == 0 ? 0 : 1)) * sizeof(cpx<T>);
+          if (lds <= MAX_BLOCK_LDS) cands.push_back(pb);
+        }
+      }
+      float best_ms = 1e30f;
+      for (int c : cands) {
+        auto run = [&] {
+          if (axis == 0) x_pass_fwd(real, true, S, ncomp, c, st);
+          else strided_pass(S, ncomp, -1, axis, c, st);
+        };
+        run();  // warm-up
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+          SCA_HIP(hipEventRecord(e0, st));
+          run();
+          SCA_HIP(hipEventRecord(e1, st));
+          SCA_HIP(hipEventSynchronize(e1));
+          float ms = 0.0f;
+          SCA_HIP(hipEventElapsedTime(&ms, e0, e1));
+          ms_min = std::min(ms_min, ms);
+        }
+        if (ms_min < best_ms) {
+          best_ms = ms_min;
+          best[axis] = c;
+        }
+      }
+    }
+  } catch (...) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (real) (void)hipFree(real);
+    if (spec) (void)hipFree(spec);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(real);
+  (void)hipFree(spec);
+  for (int a = 0; a < 3; ++a) cfg[a] = best[a];
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[key] = best;
 }
 
 template struct FftPlan<double>;
@@ -1196,8 +1394,12 @@ struct BcArgs {
   double wgt;
   int rotate;
 };
-__global__ void bc_aim_kernel(const double* sums, BcArgs b, double* aim) {
-  if (threadIdx.x != 0) return;
+struct BcBatchEntry {
+  const double* sums;
+  BcArgs b;
+  double* aim;
+};
+__device__ __forceinline__ void bc_aim_body(const double* sums, const BcArgs& b, double* aim) {
   double P[3][3], Pr[3][3], dF[3][3], t[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) P[i][j] = sums[i + 3 * j] * b.wgt;
@@ -1223,6 +1425,17 @@ __global__ void bc_aim_kernel(const double* sums, BcArgs b, double* aim) {
     for (int j = 0; j < 3; ++j)
       aim[i + 3 * j] = t[i][0] * b.R[j] + t[i][1] * b.R[3 + j] + t[i][2] * b.R[6 + j];
 }
+__global__ void bc_aim_kernel(const double* sums, BcArgs b, double* aim) {
+  if (threadIdx.x != 0) return;
+  bc_aim_body(sums, b, aim);
+}
+// one thread per RVE of a batch
+__global__ void bc_aim_batch_kernel(const BcBatchEntry* __restrict__ tab, int n) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  bc_aim_body(tab[r].sums, tab[r].b, tab[r].aim);
+}
+
 void launch_bc_aim(const double* sums, const double S[81], const double R[9], double wgt, int rotate, double* aim,
                    hipStream_t st) {
   BcArgs b;
@@ -1233,6 +1446,95 @@ void launch_bc_aim(const double* sums, const double S[81], const double R[9], do
   hipLaunchKernelGGL(bc_aim_kernel, dim3(1), dim3(64), 0, st, sums, b, aim);
   SCA_HIP(hipGetLastError());
 }
+
+TableArena::~TableArena() {
+  if (hptr) (void)hipHostFree(hptr);
+  if (dptr) (void)hipFree(dptr);
+}
+void TableArena::reserve(size_t bytes) {
+  used = 0;
+  if (bytes <= cap) return;
+  bytes = std::max(bytes, 2 * cap);
+  if (hptr) SCA_HIP(hipHostFree(hptr));
+  if (dptr) SCA_HIP(hipFree(dptr));
+  hptr = dptr = nullptr;
+  cap = 0;
+  SCA_HIP(hipHostMalloc((void**)&hptr, bytes, hipHostMallocDefault));
+  SCA_HIP(hipMalloc((void**)&dptr, bytes));
+  cap = bytes;
+}
+
+size_t batch_table_bytes_per_rve() {
+  // material point (KernelArgs and P block sums, constitutive.hip) + aim +
+  // Gamma + three copies + three reductions, with alignment slack
+  return 1024 + sizeof(BcBatchEntry) + sizeof(GammaBatchEntry) + 3 * sizeof(BatchCopy) + 3 * sizeof(BatchReduce) +
+         8 * 256;
+}
+
+void launch_final_reduce_batch(const BatchReduce* items, int n, TableArena& ta, hipStream_t st) {
+  if (n <= 0) return;
+  BatchReduce* d = nullptr;
+  BatchReduce* h = ta.add<BatchReduce>(n, &d);
+  int nv = 0;
+  for (int i = 0; i < n; ++i) {
+    h[i] = items[i];
+    nv = std::max(nv, items[i].nvec);
+  }
+  ta.upload(h, n, st);
+  hipLaunchKernelGGL(final_reduce_batch_kernel, dim3(nv, n), dim3(256), 0, st, d);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_copy_batch(const BatchCopy* items, int n, TableArena& ta, hipStream_t st) {
+  if (n <= 0) return;
+  BatchCopy* d = nullptr;
+  BatchCopy* h = ta.add<BatchCopy>(n, &d);
+  int64_t mx = 0;
+  for (int i = 0; i < n; ++i) {
+    h[i] = items[i];
+    mx = std::max(mx, items[i].n);
+  }
+  ta.upload(h, n, st);
+  const unsigned bx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (mx + 255) / 256), 1024);
+  hipLaunchKernelGGL(copy_batch_kernel, dim3(bx, n), dim3(256), 0, st, d);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_bc_aim_batch(const BcBatchItem* items, int n, TableArena& ta, hipStream_t st) {
+  if (n <= 0) return;
+  BcBatchEntry* d = nullptr;
+  BcBatchEntry* h = ta.add<BcBatchEntry>(n, &d);
+  for (int i = 0; i < n; ++i) {
+    const BcBatchItem& it = items[i];
+    h[i].sums = it.sums;
+    for (int k = 0; k < 81; ++k) h[i].b.S[k] = it.S[k];
+    for (int k = 0; k < 9; ++k) h[i].b.R[k] = it.R[k];
+    h[i].b.wgt = it.wgt;
+    h[i].b.rotate = it.rotate;
+    h[i].aim = it.aim;
+  }
+  ta.upload(h, n, st);
+  hipLaunchKernelGGL(bc_aim_batch_kernel, dim3((n + 63) / 64), dim3(64), 0, st, d, n);
+  SCA_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_gamma_div_batch(const GammaBatchItem* items, int n, int64_t K, TableArena& ta, hipStream_t st) {
+  if (n <= 0) return;
+  GammaBatchEntry* d = nullptr;
+  GammaBatchEntry* h = ta.add<GammaBatchEntry>(n, &d);
+  for (int i = 0; i < n; ++i) {
+    const GammaBatchItem& it = items[i];
+    h[i].data = it.spec;
+    h[i].g = gamma_args(it.gh, it.d_xi_tab, it.d_aim);
+    h[i].div_partial = it.div_partial;
+  }
+  ta.upload(h, n, st);
+  hipLaunchKernelGGL(gamma_div_batch_kernel<T>, dim3(gamma_blocks(K), n), dim3(256), 0, st, d, K);
+  SCA_HIP(hipGetLastError());
+}
+template void launch_gamma_div_batch<double>(const GammaBatchItem*, int, int64_t, TableArena&, hipStream_t);
+template void launch_gamma_div_batch<float>(const GammaBatchItem*, int, int64_t, TableArena&, hipStream_t);
 
 void launch_final_reduce(const double* partial, int nvec, int nb, double* out, hipStream_t st) {
   hipLaunchKernelGGL(final_reduce_kernel, dim3(nvec), dim3(256), 0, st, partial, nb, out);

@@ -38,10 +38,17 @@ PRODUCT_SO = os.path.join(ROOT, "solver_cpp_amd", "_build", "libsolver_cpp_amd.s
 class Server:
     """A job server of the HIP library."""
 
-    def __init__(self, lib: Library, hbm_budget_bytes: int, workers: int):
+    def __init__(self, lib: Library, hbm_budget_bytes: int, workers: int, batched: bool = False,
+                 max_wait_ms: float = 5.0):
+        """batched: the residual evaluations of the running jobs are computed
+        in joint launches (sc_server_create_batched)."""
         self.lib = lib
         h = C.c_void_p()
-        lib.check(lib.sc_server_create(int(hbm_budget_bytes), int(workers), C.byref(h)), "sc_server_create")
+        if batched:
+            lib.check(lib.sc_server_create_batched(int(hbm_budget_bytes), int(workers), float(max_wait_ms),
+                                                   C.byref(h)), "sc_server_create_batched")
+        else:
+            lib.check(lib.sc_server_create(int(hbm_budget_bytes), int(workers), C.byref(h)), "sc_server_create")
         self.h = h
         self._keep = {}
 
@@ -67,6 +74,16 @@ class Server:
 
     def remove_job(self, job: int):
         self.lib.check(self.lib.sc_server_remove_job(self.h, job), "sc_server_remove_job")
+
+    def field(self, job: int, name: str, n_cells: int) -> np.ndarray:
+        """A field of an idle resident job, as Rve.field (sc_server_get_field)."""
+        from .capi import SC_FIELD
+
+        n = {"xi": 12, "gamma": 12, "x_p": 3, "iterations": 2}.get(name, 9) * int(n_cells)
+        out = np.zeros(n)
+        self.lib.check(self.lib.sc_server_get_field(self.h, job, SC_FIELD[name], out.ctypes.data_as(
+            C.POINTER(C.c_double)), n), f"sc_server_get_field {name}")
+        return out
 
     def stats(self) -> ServerStats:
         s = ServerStats()
@@ -126,10 +143,12 @@ def batched_specs(lib, n_rve: int, seed: int = 7, edges=(16, 24, 32, 48), grains
     return out
 
 
-def run_batched(lib, specs, workers: int, budget: int, t: float = 5.0, N: int = 10, increments: int = 0):
+def run_batched(lib, specs, workers: int, budget: int, t: float = 5.0, N: int = 10, increments: int = 0,
+                batched: bool = False, max_wait_ms: float = 5.0):
     """Every RVE through its load case (the first `increments` of its N
-    increments when given) on the server; returns totals."""
-    srv = Server(lib, budget, workers)
+    increments when given) on the server; returns totals.  batched: joint
+    launches of the running jobs' residual evaluations."""
+    srv = Server(lib, budget, workers, batched=batched, max_wait_ms=max_wait_ms)
     t0 = time.perf_counter()
     reqs = []
     n_inc = increments if increments > 0 else N
@@ -143,8 +162,10 @@ def run_batched(lib, specs, workers: int, budget: int, t: float = 5.0, N: int = 
     srv.close()
     updates = sum(r.material_point_updates for rs in results for r in rs)
     bad = sum(1 for rs in results for r in rs if r.status != 0)
+    lat = np.array([r.t_end - r.t_start for rs in results for r in rs])
     return dict(wall=wall, updates=updates, value=updates / wall, failed=bad, max_concurrent=st.max_concurrent,
-                peak_bytes=st.peak_bytes, results=results)
+                peak_bytes=st.peak_bytes, results=results, batches=st.batches, max_batch=st.max_batch,
+                batched_evaluations=st.batched_evaluations, increment_ms=lat * 1e3)
 
 
 def serving_requests(lib, n_req: int, rate: float, seed: int = 11, edge_range=(16, 128), precision: int = 0,

@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Oracle fixtures of whole load cases, for the GPU parity tests.
+
+The CPU oracle (oracle/_build/liboracle.so, the FP64 restatement of the
+reference path) is run here, in the build container, on the drop-in and
+batched configurations; the GPU tests then compare the HIP library against
+the committed results without re-running the oracle.
+
+    python tests/golden/make_load_fixtures.py c2 [--dump DIR]
+        64^3 grid, 200 Voronoi grains (seeds 3/4), tension to 2 % in 40
+        increments (about 35 min on 8 cores).  With --dump, the full F, P,
+        Fp and xi fields of the sampled increments are written to DIR.
+    python tests/golden/make_load_fixtures.py c2-pack --dump DIR [--extra-cells FILE]
+        c2_fixture.npz from a dump: per-increment iterations, evaluations
+        and <P>, plus F, P, Fp and xi of a fixed cell sample at the sampled
+        increments (4096 seeded random cells and the cells listed in FILE).
+    python tests/golden/make_load_fixtures.py c4
+        c4_fixture.npz: RVEs of the batched workload's generator (master
+        seed 7; cubic edges 16, 24, 32 and 48, 10-200 grains, tension along
+        a random direction, 10 increments of 0.5 s), per-increment counts
+        and <P> and the final P and F on 512 sampled cells of each RVE.
+    python tests/golden/make_load_fixtures.py c5
+        c5_fixture.npz: the first requests of the serving stream (seed 11,
+        edges log-uniform in [16, 128] rounded to 2^a 3^b) whose grids have
+        at most 36^3 cells: one increment of 1e-4 strain each, counts, <P>
+        and P on 512 sampled cells.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from solver_cpp_amd.capi import Library  # noqa: E402
+from solver_cpp_amd.geometry import CONFIGS, EXAMPLE_NUMERICS, polycrystal  # noqa: E402
+from solver_cpp_amd.rve import Rve, tension_x  # noqa: E402
+
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+C2_SAMPLED = (2, 10, 20, 40)
+C2_FIELDS = ("F", "P", "Fp", "xi")
+C4_RVES = 8
+C4_SAMPLE = 512
+C5_REQUESTS = 24
+C5_MAX_EDGE = 36
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def c2_run(dump: str | None):
+    lib = Library(ORACLE_SO)
+    c = CONFIGS["C2"]
+    cells = (c["edge"],) * 3
+    mat, ori, size = polycrystal(cells, c["grains"], seed=c["seed"], ori_seed=c["ori_seed"])
+    r = Rve(lib, cells, size, mat, ori, **EXAMPLE_NUMERICS)
+    r.init()
+    its, evs, pav = [], [], []
+    t0 = time.time()
+    for inc in r.increments([tension_x(t=c["t"], N=c["N"])]):
+        its.append(inc["iterations"])
+        evs.append(inc["evaluations"])
+        pav.append(np.array(r.scalars().P_av))
+        log(f"c2 increment {inc['increment']}: {inc['iterations']} iterations, {inc['evaluations']} evaluations, "
+            f"{time.time() - t0:.0f} s")
+        if dump and inc["increment"] in C2_SAMPLED:
+            os.makedirs(dump, exist_ok=True)
+            np.savez(os.path.join(dump, f"inc{inc['increment']}.npz"), **{k: r.field(k) for k in C2_FIELDS})
+    if dump:
+        np.savez(os.path.join(dump, "counts.npz"), iterations=np.array(its), evaluations=np.array(evs),
+                 P_av=np.array(pav))
+    return np.array(its), np.array(evs), np.array(pav)
+
+
+def c2_pack(dump: str, extra_cells: str | None):
+    c = CONFIGS["C2"]
+    N = c["edge"] ** 3
+    cnt = np.load(os.path.join(dump, "counts.npz"))
+    rng = np.random.default_rng(2024)
+    cells = set(rng.choice(N, 4096, replace=False).tolist())
+    if extra_cells:
+        cells |= {int(x) for x in open(extra_cells).read().split()}
+    cells = np.array(sorted(cells), dtype=np.int64)
+    out = dict(iterations=cnt["iterations"], evaluations=cnt["evaluations"], P_av=cnt["P_av"], cells=cells,
+               sampled=np.array(C2_SAMPLED))
+    for inc in C2_SAMPLED:
+        d = np.load(os.path.join(dump, f"inc{inc}.npz"))
+        for k in C2_FIELDS:
+            comps = d[k].size // N
+            # fields come component fastest ((3,3,N), (12,N)): rows of reshape(N, comps) are cells
+            out[f"{k}_{inc}"] = d[k].reshape(N, comps)[cells]
+    path = os.path.join(HERE, "c2_fixture.npz")
+    np.savez_compressed(path, **out)
+    log(f"wrote {path} ({os.path.getsize(path)} bytes, {cells.size} cells)")
+
+
+def _sample(N: int, seed: int, n: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(N, min(n, N), replace=False))
+
+
+def c4_specs(lib):
+    """The first RVEs of the batched generator that cover every edge."""
+    from solver_cpp_amd.serving import batched_specs
+
+    specs = batched_specs(lib, 64, seed=7, **EXAMPLE_NUMERICS)
+    chosen, seen = [], {}
+    for i, (spec, q) in enumerate(specs):
+        e = spec.cells[0]
+        if seen.get(e, 0) < 2:
+            seen[e] = seen.get(e, 0) + 1
+            chosen.append(i)
+        if len(chosen) == C4_RVES:
+            break
+    return specs, chosen
+
+
+def c4_run():
+    from solver_cpp_amd.rve import Rve as _Rve
+
+    lib = Library(ORACLE_SO)
+    specs, chosen = c4_specs(lib)
+    out = dict(index=np.array(chosen))
+    t0 = time.time()
+    for j, i in enumerate(chosen):
+        spec, q = specs[i]
+        r = _Rve(lib, spec.cells, [spec.desc.size[k] for k in range(3)], spec._mat, spec._ori, **EXAMPLE_NUMERICS)
+        r.init()
+        its, evs, pav = [], [], []
+        for inc in r.increments([tension_x(t=5.0, N=10, rot_q=q)]):
+            its.append(inc["iterations"])
+            evs.append(inc["evaluations"])
+            pav.append(np.array(r.scalars().P_av))
+        cells = _sample(r.N, 100 + i, C4_SAMPLE)
+        out[f"iterations_{j}"] = np.array(its)
+        out[f"evaluations_{j}"] = np.array(evs)
+        out[f"P_av_{j}"] = np.array(pav)
+        out[f"cells_{j}"] = cells
+        out[f"P_{j}"] = r.field("P").reshape(r.N, 9)[cells]
+        out[f"F_{j}"] = r.field("F").reshape(r.N, 9)[cells]
+        log(f"c4 rve {i} edge {spec.cells[0]}: iterations {its}, {time.time() - t0:.0f} s")
+        r.close()
+    path = os.path.join(HERE, "c4_fixture.npz")
+    np.savez_compressed(path, **out)
+    log(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def c5_requests(lib):
+    from solver_cpp_amd.serving import serving_requests
+
+    reqs = serving_requests(lib, C5_REQUESTS, rate=64.0, seed=11, edge_range=(16, 128), **EXAMPLE_NUMERICS)
+    small = [i for i, (_, spec) in enumerate(reqs) if spec.cells[0] <= C5_MAX_EDGE]
+    return reqs, small
+
+
+def c5_run():
+    from solver_cpp_amd.rve import Rve as _Rve
+
+    lib = Library(ORACLE_SO)
+    reqs, small = c5_requests(lib)
+    out = dict(index=np.array(small), edges=np.array([spec.cells[0] for _, spec in reqs]))
+    for i in small:
+        spec = reqs[i][1]
+        r = _Rve(lib, spec.cells, [spec.desc.size[k] for k in range(3)], spec._mat, spec._ori, **EXAMPLE_NUMERICS)
+        r.init()
+        inc = next(r.increments([tension_x(t=0.1, N=1)]))
+        cells = _sample(r.N, 200 + i, C4_SAMPLE)
+        out[f"iterations_{i}"] = np.array(inc["iterations"])
+        out[f"evaluations_{i}"] = np.array(inc["evaluations"])
+        out[f"P_av_{i}"] = np.array(r.scalars().P_av)
+        out[f"cells_{i}"] = cells
+        out[f"P_{i}"] = r.field("P").reshape(r.N, 9)[cells]
+        log(f"c5 request {i} edge {spec.cells[0]}: {inc['iterations']} iterations")
+        r.close()
+    path = os.path.join(HERE, "c5_fixture.npz")
+    np.savez_compressed(path, **out)
+    log(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("what", choices=["c2", "c2-pack", "c4", "c5"])
+    ap.add_argument("--dump")
+    ap.add_argument("--extra-cells")
+    a = ap.parse_args()
+    if a.what == "c2":
+        c2_run(a.dump)
+        if a.dump:
+            c2_pack(a.dump, a.extra_cells)
+    elif a.what == "c2-pack":
+        c2_pack(a.dump, a.extra_cells)
+    elif a.what == "c4":
+        c4_run()
+    else:
+        c5_run()
+
+
+if __name__ == "__main__":
+    main()

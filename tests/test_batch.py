@@ -1,0 +1,116 @@
+"""Batched residual evaluations (sc_server_create_batched): the evaluations
+of all running jobs are computed in joint launches -- one material-point
+launch over every job's cells, one batched FFT per pass and one Gamma launch
+per group of equal grids.  Each job's arithmetic is unchanged, so every
+result must be bitwise the one of the same job run alone on its own stream,
+and the workload's oracle fixtures hold for the batched form as well."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, PRODUCT_SO
+from solver_cpp_amd.capi import Library
+from solver_cpp_amd.geometry import EXAMPLE_NUMERICS
+from solver_cpp_amd.rve import tension_x
+from solver_cpp_amd.serving import Server, batched_specs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return Library(PRODUCT_SO, boundary=True)
+
+
+def _run(lib, specs, loads, incs, batched, workers):
+    srv = Server(lib, 1 << 37, workers, batched=batched, max_wait_ms=20.0)
+    jobs = [srv.add_job(s, l, keep_resident=True) for (s, _), l in zip(specs, loads)]
+    reqs = [[srv.submit(j) for _ in range(incs)] for j in jobs]
+    res = [[srv.wait(r, 300.0) for r in rs] for rs in reqs]
+    fields = [{k: srv.field(j, k, s.N) for k in ("F", "P", "Fp", "xi")} for j, (s, _) in zip(jobs, specs)]
+    st = srv.stats()
+    for j in jobs:
+        srv.remove_job(j)
+    srv.close()
+    return res, fields, st
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batched_equals_alone(lib, precision):
+    """Eight RVEs of uneven grids (8, 12, 16, two of them of one size) and
+    random tension directions, three increments each: alone (one worker, one
+    job at a time) and batched (eight workers) give bitwise the same counts,
+    <P> and fields, and the batched run did join evaluations."""
+    specs = batched_specs(lib, 8, seed=9, edges=(8, 12, 16), grains=(3, 12), precision=precision,
+                          **EXAMPLE_NUMERICS)
+    loads = [[tension_x(t=1.5, N=3, rot_q=q)] for _, q in specs]
+    print(f"[batch] precision {precision}: edges {[s.cells[0] for s, _ in specs]}", flush=True)
+    alone, fa, _ = _run(lib, specs, loads, 3, batched=False, workers=1)
+    batch, fb, st = _run(lib, specs, loads, 3, batched=True, workers=8)
+    assert st.batches > 0 and st.max_batch >= 2, (st.batches, st.max_batch)
+    for a_job, b_job in zip(alone, batch):
+        for a, b in zip(a_job, b_job):
+            assert a.status == 0 and b.status == 0
+            assert (a.iterations, a.evaluations, a.increment) == (b.iterations, b.evaluations, b.increment)
+            np.testing.assert_array_equal(np.array(a.P_av), np.array(b.P_av))
+    for x, y in zip(fa, fb):
+        for k in x:
+            np.testing.assert_array_equal(x[k], y[k], err_msg=k)
+
+
+C4_FIXTURE = os.path.join(GOLDEN, "c4_fixture.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(C4_FIXTURE), reason="c4 fixture not generated")
+def test_c4_batched_against_oracle(lib):
+    """The batched workload's RVEs (generator seed 7, edges 16, 24, 32, 48,
+    10-200 grains, tension along random directions, 10 increments of 0.5 s)
+    run together through the batched server under a memory budget that
+    admits only some of them at once, against the oracle's runs of the same
+    RVEs: identical iterations and evaluations in every increment, <P>
+    within 1e-10 relative at every increment, P and F within 1e-10 on the
+    sampled cells after the last increment."""
+    fx = np.load(C4_FIXTURE)
+    specs = batched_specs(lib, 64, seed=7, **EXAMPLE_NUMERICS)
+    chosen = [specs[i] for i in fx["index"]]
+    print(f"[c4] edges {[s.cells[0] for s, _ in chosen]}", flush=True)
+    fps = [lib.sc_rve_footprint(C.byref(s.desc)) for s, _ in chosen]
+    # counts and <P> with about four jobs admitted at once (the others wait
+    # for memory; a job's memory is released after its last increment)
+    budget = int(sum(sorted(fps)[-4:]))
+    srv = Server(lib, budget, len(chosen), batched=True, max_wait_ms=20.0)
+    jobs = [srv.add_job(s, [tension_x(t=5.0, N=10, rot_q=q)]) for s, q in chosen]
+    reqs = [[srv.submit(j) for _ in range(10)] for j in jobs]
+    res = [[srv.wait(r, 600.0) for r in rs] for rs in reqs]
+    st = srv.stats()
+    srv.close()
+    assert st.peak_bytes <= budget and st.max_batch >= 2
+    # fields: all jobs resident, kept after their last increment
+    srv = Server(lib, 1 << 38, len(chosen), batched=True, max_wait_ms=20.0)
+    jobs = [srv.add_job(s, [tension_x(t=5.0, N=10, rot_q=q)], keep_resident=True) for s, q in chosen]
+    reqs = [[srv.submit(j) for _ in range(10)] for j in jobs]
+    res2 = [[srv.wait(r, 600.0) for r in rs] for rs in reqs]
+    fields = [{k: srv.field(j, k, s.N) for k in ("F", "P")} for j, (s, _) in zip(jobs, chosen)]
+    for j in jobs:
+        srv.remove_job(j)
+    srv.close()
+    for a_job, b_job in zip(res, res2):  # any admission order, the same results
+        for a, b in zip(a_job, b_job):
+            np.testing.assert_array_equal(np.array(a.P_av), np.array(b.P_av))
+    for j, (spec, _) in enumerate(chosen):
+        its = np.array([r.iterations for r in res[j]])
+        evs = np.array([r.evaluations for r in res[j]])
+        assert all(r.status == 0 for r in res[j])
+        np.testing.assert_array_equal(its, fx[f"iterations_{j}"], err_msg=f"rve {j} iterations")
+        np.testing.assert_array_equal(evs, fx[f"evaluations_{j}"], err_msg=f"rve {j} evaluations")
+        pav = np.array([r.P_av for r in res[j]])
+        ref = fx[f"P_av_{j}"]
+        assert np.abs(pav - ref).max() / np.abs(ref).max() < 1e-10, f"rve {j} <P>"
+        cells = fx[f"cells_{j}"]
+        for k in ("P", "F"):
+            got = fields[j][k].reshape(spec.N, 9)[cells]
+            want = fx[f"{k}_{j}"]
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err < 1e-10, f"rve {j} {k}: {err:.2e}"

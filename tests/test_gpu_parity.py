@@ -377,3 +377,61 @@ def test_mixed_precision_16cubed(product, oracle):
     pa, pb = np.array(a.scalars().P_av), np.array(b.scalars().P_av)
     close(pa, pb, 2e-5, "<P>")
     close(a.field("P"), b.field("P"), 1e-4, "P")
+
+
+def test_mixed_precision_tight_tolerances(product, oracle):
+    """Precision error apart from the solvers' stopping spread: with the
+    divergence and stress tolerances 1000 times tighter (eps_rel_div 5e-7,
+    eps_rel_P 1e-6, absolute floors out of the way), both forms converge to
+    the equilibrium itself, so what remains between the mixed form and the
+    FP64 oracle is the arithmetic.  The FP32 stage stops at its own floor
+    (a relative divergence of 1e-5) and the FP64 solve finishes each
+    increment.  Required: <P> within 1e-6 relative in every increment and
+    every cell's P within 1e-6 of max |P|.  The plain FP64 form is checked
+    against the same oracle run: identical counts, and P within 1e-9 (at
+    these tolerances N-GMRES works on residuals a thousand times smaller,
+    whose least-squares combinations carry rounding further than at the
+    default ones)."""
+    cfg = CONFIGS["C1"]
+    e = cfg["edge"]
+    tight = dict(itmin=4, itmax=100, eps_rel_div=5e-7, eps_abs_div=1e-12, eps_rel_P=1e-6, eps_abs_P=1e-6)
+    mat, ori, size = polycrystal((e, e, e), cfg["grains"], seed=cfg["seed"], ori_seed=cfg["ori_seed"])
+    m = Rve(product, (e, e, e), size, mat, ori, precision=1, **tight)
+    p = Rve(product, (e, e, e), size, mat, ori, precision=0, **tight)
+    o = Rve(oracle, (e, e, e), size, mat, ori, **tight)
+    for r in (m, p, o):
+        r.init()
+    load = [tension_x(t=cfg["t"], N=cfg["N"])]
+    gens = [r.increments(load) for r in (m, p, o)]
+    worst = 0.0
+    for k in range(6):
+        dm, dp, do = (next(g) for g in gens)
+        print(f"tight increment {k + 1}: mixed {dm['iterations']}/{dm['evaluations']}, fp64 {dp['iterations']}/"
+              f"{dp['evaluations']}, oracle {do['iterations']}/{do['evaluations']}", flush=True)
+        assert (dp["iterations"], dp["evaluations"]) == (do["iterations"], do["evaluations"])
+        pm, po = np.array(m.scalars().P_av), np.array(o.scalars().P_av)
+        err = np.abs(pm - po).max() / np.abs(po).max()
+        worst = max(worst, err)
+        assert err <= 1e-6, f"increment {k + 1}: <P> {err:.2e}"
+        close(p.field("P"), o.field("P"), 1e-9, "P (fp64)")
+    print(f"tight: worst <P> error of the mixed form {worst:.2e}", flush=True)
+    close(m.field("P"), o.field("P"), 1e-6, "P (mixed)")
+    st = m.stats()
+    assert st.fp32_evaluations > 0
+
+
+def test_mixed_precision_reuses_the_last_update(product):
+    """The FP64 finish of a mixed increment starts at the F the FP32 stage
+    returned; when that was the FP32 stage's last evaluation, its FP64
+    material-point update is taken as it is: evaluations outnumber the
+    material-point updates, and the FP64 spectral check still runs in every
+    increment."""
+    cells = (16, 16, 16)
+    mat, ori, size = polycrystal(cells, 20, seed=1, ori_seed=2)
+    r = Rve(product, cells, size, mat, ori, precision=1, **EXAMPLE_NUMERICS)
+    r.init()
+    r.reset_stats()
+    n_inc = len(list(r.increments([tension_x(t=3.0, N=6)])))
+    st = r.stats()
+    assert st.residual_evaluations - st.fp32_evaluations >= n_inc
+    assert st.material_point_updates < st.residual_evaluations * r.N

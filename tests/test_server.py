@@ -34,6 +34,24 @@ def test_footprint_and_budget(lib):
     srv.close()
 
 
+def test_invalid_job_is_refused_and_holds_no_memory(lib):
+    """(CPU) A job that could not be started -- a material index out of range,
+    a grid length with a prime factor above 13 -- is refused when it is
+    added, with the reason, and leaves the memory accounting at zero."""
+    srv = Server(lib, 1 << 34, 2)
+    bad = _spec(lib, 8, 3, 1)
+    bad._mat[5] = 7  # only 3 materials
+    with pytest.raises(RuntimeError, match="material_at out of range"):
+        srv.add_job(bad, [tension_x(t=1.0, N=2)])
+    mat, ori, size = polycrystal((17, 8, 8), 3, seed=2)
+    odd = RveSpec(lib, (17, 8, 8), size, mat, ori)
+    with pytest.raises(RuntimeError, match="prime factor"):
+        srv.add_job(odd, [tension_x(t=1.0, N=2)])
+    st = srv.stats()
+    assert st.used_bytes == 0 and st.resident_jobs == 0 and st.peak_bytes == 0
+    srv.close()
+
+
 def test_rotation_to():
     for u in ([0, 0, 1], [1, 1, 1], [-1, 0.2, 0.1]):
         q = np.array(rotation_to(u))
