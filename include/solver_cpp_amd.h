@@ -412,6 +412,32 @@ int sc_mp_update(int64_t n, double dt, const sc_material* m, const sc_phase_nume
                  double* Lp, double* P, double* Fp, double* S, double* Fe,
                  double* xi, double* gamma, double* dPdF, int* broken, int* iterations);
 
+/* Material-point update without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The update of sc_mp_update, enqueued on the caller's stream (NULL: the null
+ * stream) on device arrays the caller owns, in structure-of-arrays layout:
+ * a (3,3) field is 9 planes of n doubles (plane q = i+3j holds component
+ * (i,j) of every point), a slip field 12 planes of n doubles, dPdF 81 planes
+ * (plane i+3j+9k+27l), iterations 2 planes of n ints (Newton, state
+ * iterations) and broken n ints.  The substep scratch, the pivoting pass's
+ * point list and the terminally-ill flag live in one fixed device array of
+ * the library sized for SC_MP_NOALLOC_MAX_POINTS points (about 16 MB, held
+ * on each GPU the library runs on until the process ends); larger n is
+ * refused with an error.  The call allocates nothing.  Calls that overlap on
+ * one GPU share that array and race: serialise them. */
+#define SC_MP_NOALLOC_MAX_POINTS 32768
+typedef struct sc_mp_device_planes {
+  const double *F, *F0, *Fp0, *Lp0, *S0, *xi0, *gamma0;  /* inputs, as in sc_mp_update */
+  double* Lp;                                            /* in: Newton guess; out: Lp */
+  double *P, *Fp, *S, *Fe, *xi, *gamma;                  /* outputs */
+  double* dPdF;                                          /* may be NULL */
+  int* broken;                                           /* may be NULL */
+  int* iterations;                                       /* may be NULL */
+} sc_mp_device_planes;
+int sc_mp_update_noalloc(int64_t n, double dt, const sc_material* m, const sc_phase_numerics* num,
+                         const sc_mp_device_planes* planes, void* stream);
+
 /* Orientation matrix of a unit quaternion, qu2om with P = -1
  * (rotations.f90:423-447); out (3,3) column-major. */
 void sc_qu2om(const double q[4], double om[9]);

@@ -109,6 +109,13 @@ class ServerStats(C.Structure):
     ]
 
 
+class MpDevicePlanes(C.Structure):
+    """sc_mp_device_planes: device addresses of the structure-of-arrays planes
+    of sc_mp_update_noalloc (0 for an optional array left out)."""
+    _fields_ = [(k, C.c_void_p) for k in ("F", "F0", "Fp0", "Lp0", "S0", "xi0", "gamma0", "Lp", "P", "Fp", "S",
+                                          "Fe", "xi", "gamma", "dPdF", "broken", "iterations")]
+
+
 KT_CLASSES = ["response", "tangent", "fft_fwd", "fft_bwd", "gamma", "reduce", "other"]
 
 
@@ -182,6 +189,8 @@ PROFILE_SIGNATURES = {
     "sc_dadf5_attr_str": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p]),
     "sc_dadf5_attr_f64": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_double]),
     "sc_dadf5_close": (C.c_int, [C.c_void_p]),
+    "sc_mp_update_noalloc": (C.c_int, [C.c_int64, C.c_double, C.POINTER(Material), C.POINTER(PhaseNumerics),
+                                       C.POINTER(MpDevicePlanes), C.c_void_p]),
 }
 
 
