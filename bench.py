@@ -13,8 +13,9 @@ plus the p50 / p99 increment latency.  The plain FP64 form runs the same
 increments for the comparison; the equal-work comparison is the increment
 time.  A second part runs the batched workload (256 RVEs of uneven grids
 through their load cases) through the batched job server, in both
-precisions, against the same RVEs run one at a time in FP64.  One JSON line
-is printed on stdout.
+precisions, against the same RVEs run one at a time in FP64, and a third
+the serving stream (Poisson requests of one increment on RVEs of edges 16 to
+128) at several arrival rates.  One JSON line is printed on stdout.
 
     python bench.py [--steps K] [--warmup W] [--precision fp64|mixed]
 """
@@ -159,10 +160,45 @@ def run_batched_workload(args, precision: str, batched: bool, workers: int):
            "peak_gb": r["peak_bytes"] / 2 ** 30}
     if batched:
         out.update(batches=r["batches"], max_batch=r["max_batch"],
-                   mean_batch=r["batched_evaluations"] / max(r["batches"], 1))
+                   mean_batch=r["batched_evaluations"] / max(r["batches"], 1),
+                   joint_launch_seconds=r["batch_seconds"])
     P = [np.array(x.P_av) for rs in r["results"] for x in rs]
     del specs
     return out, P
+
+
+def run_serving_sweep(args):
+    """The serving stream: Poisson requests, each one 1e-4 strain increment on
+    its own RVE (edges log-uniform in [16, 128], rounded to FFT-friendly
+    lengths), through the batched job server at several arrival rates, and
+    the same requests one at a time (one worker, no joint launches)."""
+    from solver_cpp_amd.serving import run_one_at_a_time, run_serving, serving_requests
+
+    lib = Library(PRODUCT_SO, boundary=True)
+    rates = [float(x) for x in args.serving_rates.split(",")]
+    out = {"workload": f"{args.serving_requests} Poisson requests (generator seed 11), each one increment of "
+                       f"1e-4 strain (tension_x, 0.1 s) on its own RVE: cubic grid, edge log-uniform in [16, 128] "
+                       f"rounded to 2^a 3^b, 10..min(1000, cells/8) Voronoi grains; batched server, 16 workers",
+           "rates": []}
+    seq = None
+    for rate in rates:
+        reqs = serving_requests(lib, args.serving_requests, rate, **EXAMPLE_NUMERICS)
+        r = run_serving(lib, reqs, 16, int(200 * (1 << 30)), batched=True, max_wait_ms=2.0)
+        if seq is None:
+            seq = run_one_at_a_time(lib, reqs)
+            out["one_at_a_time"] = {"p50_ms": float(np.percentile(seq["lat"], 50) * 1e3),
+                                    "p99_ms": float(np.percentile(seq["lat"], 99) * 1e3),
+                                    "mat_point_updates_per_s": r["updates"] / seq["wall"]}
+        same = all(np.array_equal(a, b) for a, b in zip(r["P_av"], seq["P_av"]))
+        row = {"rate_per_s": rate, "p50_ms": float(np.percentile(r["lat"], 50) * 1e3),
+               "p99_ms": float(np.percentile(r["lat"], 99) * 1e3), "mat_point_updates_per_s": r["value"],
+               "max_concurrent": r["max_concurrent"], "batches": r["batches"], "max_batch": r["max_batch"],
+               "failed": r["failed"], "bitwise_equal_to_one_at_a_time": bool(same)}
+        log(f"[bench] serving at {rate:g} req/s: p50 {row['p50_ms']:.1f} ms, p99 {row['p99_ms']:.1f} ms, "
+            f"{row['mat_point_updates_per_s']:.3e} updates/s, max concurrent {row['max_concurrent']}")
+        out["rates"].append(row)
+        del reqs
+    return out
 
 
 def cpu_info():
@@ -274,6 +310,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--batched-rves", type=int, default=256)
     ap.add_argument("--no-tight", action="store_true", help="skip the tight-tolerance precision comparison")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--serving-rates", default="32,128,512", help="serving sweep: arrival rates per second")
+    ap.add_argument("--serving-requests", type=int, default=64)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -365,6 +404,8 @@ def main():
             "speedup_fp64_vs_plain_one_at_a_time": bf["value"] / one["value"],
             "fp64_bitwise_equal_to_one_at_a_time": bool(same),
             "mixed_max_P_av_rel_diff_vs_fp64": err}
+    if not args.no_serving:
+        line["serving"] = run_serving_sweep(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args, loads)
     print(json.dumps(line), flush=True)

@@ -310,6 +310,7 @@ typedef struct sc_server_stats {
   int64_t batches;                /* joint launches of residual evaluations (batched server) */
   int64_t batched_evaluations;    /* residual evaluations computed in them */
   int64_t max_batch;              /* most evaluations in one joint launch */
+  double  batch_seconds;          /* wall time of the joint launches (completion included) */
 } sc_server_stats;
 
 /* workers: host threads, each driving one admitted job at a time on its own

@@ -105,7 +105,7 @@ class ServerStats(C.Structure):
         ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64), ("peak_bytes", C.c_int64),
         ("resident_jobs", C.c_int64), ("completed_requests", C.c_int64), ("max_concurrent", C.c_int),
         ("evictions", C.c_int64), ("batches", C.c_int64), ("batched_evaluations", C.c_int64),
-        ("max_batch", C.c_int64),
+        ("max_batch", C.c_int64), ("batch_seconds", C.c_double),
     ]
 
 

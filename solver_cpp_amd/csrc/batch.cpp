@@ -33,7 +33,7 @@
 namespace sca {
 
 Batcher::Batcher(double max_wait_ms) : max_wait_(max_wait_ms) {
-  SCA_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+  st_ = stream_get();
   th_ = std::thread([this] { run(); });
 }
 
@@ -48,12 +48,12 @@ Batcher::~Batcher() {
     Group& g = kv.second;
     g.f64.release();
     g.f32.release();
-    if (g.real) (void)hipFree(g.real);
-    if (g.spec) (void)hipFree(g.spec);
+    dev_free(g.real);
+    dev_free(g.spec);
   }
-  if (d_scal_) (void)hipFree(d_scal_);
-  if (h_scal_) (void)hipHostFree(h_scal_);
-  if (st_) (void)hipStreamDestroy(st_);
+  dev_free(d_scal_);
+  host_free(h_scal_);
+  stream_put(st_);
 }
 
 void Batcher::join() {
@@ -104,14 +104,17 @@ void Batcher::run() {
       take.swap(waiting_);
     }
     std::string err;
+    const auto t0 = std::chrono::steady_clock::now();
     try {
       launch(take);
     } catch (std::exception& e) {
       err = e.what();
       (void)hipGetLastError();
     }
+    const double busy = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     {
       std::lock_guard<std::mutex> lk(m_);
+      stats_.busy_seconds += busy;
       for (Req* q : take) {
         q->err = err;
         q->done = true;
@@ -130,13 +133,13 @@ void Batcher::ensure(Group& g, int cap) {
   if (g.cap >= cap) return;
   const int64_t N = (int64_t)g.cells[0] * g.cells[1] * g.cells[2];
   const int64_t K = (int64_t)(g.cells[0] / 2 + 1) * g.cells[1] * g.cells[2];
-  if (g.real) SCA_HIP(hipFree(g.real));
-  if (g.spec) SCA_HIP(hipFree(g.spec));
+  dev_free(g.real);  // the batch that used them has completed
+  dev_free(g.spec);
   g.real = nullptr;
   g.spec = nullptr;
   g.cap = 0;
-  SCA_HIP(hipMalloc(&g.real, (size_t)9 * N * cap * sizeof(double)));
-  SCA_HIP(hipMalloc(&g.spec, (size_t)2 * 9 * K * cap * (g.fp32 ? sizeof(float) : sizeof(double))));
+  g.real = static_cast<double*>(dev_alloc((size_t)9 * N * cap * sizeof(double)));
+  g.spec = dev_alloc((size_t)2 * 9 * K * cap * (g.fp32 ? sizeof(float) : sizeof(double)));
   g.cap = cap;
 }
 
@@ -233,13 +236,13 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
   constexpr int NS = 11;
   if (scal_cap_ < n) {
     const int cap = std::max(n, 2 * scal_cap_);
-    if (d_scal_) SCA_HIP(hipFree(d_scal_));
-    if (h_scal_) SCA_HIP(hipHostFree(h_scal_));
+    dev_free(d_scal_);
+    host_free(h_scal_);
     d_scal_ = nullptr;
     h_scal_ = nullptr;
     scal_cap_ = 0;
-    SCA_HIP(hipMalloc(&d_scal_, (size_t)NS * cap * sizeof(double)));
-    SCA_HIP(hipHostMalloc((void**)&h_scal_, (size_t)NS * cap * sizeof(double), hipHostMallocDefault));
+    d_scal_ = static_cast<double*>(dev_alloc((size_t)NS * cap * sizeof(double)));
+    h_scal_ = static_cast<double*>(host_alloc((size_t)NS * cap * sizeof(double)));
     scal_cap_ = cap;
   }
   std::vector<BatchCopy> sc(n);

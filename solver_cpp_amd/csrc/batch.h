@@ -16,6 +16,7 @@ namespace sca {
 
 struct BatchStats {
   int64_t batches = 0, evaluations = 0, max_batch = 0;
+  double busy_seconds = 0.0;  // wall time of the joint launches, their completion included
 };
 
 // Gathers the residual evaluations of the RVEs that are inside an increment

@@ -508,6 +508,7 @@ int sc_rve_set_field(sc_rve* R, int field, const double* in, int64_t n) {
     a.put(in, v.st);
     sca::launch_aos_to_soa(a.p, v.X, v.N, 9, v.st);
     SCA_HIP(hipStreamSynchronize(v.st));
+    v.x_uniform = false;
     return 0;
   });
 }

@@ -18,6 +18,20 @@ namespace sca {
                                __FILE__ + ":" + std::to_string(__LINE__));                \
   } while (0)
 
+// Caching allocators (pool.cpp): device and pinned host blocks the library
+// returns are kept per rounded size and handed out again, and streams are
+// reused, so starting and releasing RVEs calls neither hipMalloc nor hipFree
+// (which synchronise the whole device) once blocks of their sizes exist.  A
+// block may be returned only after the work that used it has completed.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+void* host_alloc(size_t bytes);
+void host_free(void* p);
+void pool_trim(size_t keep_bytes);  // hipFree cached device blocks beyond keep_bytes
+size_t pool_cached_bytes();
+hipStream_t stream_get();  // a non-blocking stream of default priority
+void stream_put(hipStream_t s);
+
 constexpr int NSLIP = 12;
 constexpr int NSTATE = 24;
 

@@ -16,24 +16,31 @@
 // changes no arithmetic.  The twiddle table (length N) is staged in LDS
 // with the data.
 //
-// Supported: N = 2^a (8..512, E = 8) and N = 3 * 2^a (12..384, E = 12);
-// other lengths use the generic Stockham kernels.
+// Supported: N = 2^a (8..512, E = 8), N = 3 * 2^a (12..384, E = 12),
+// N = 9 * 2^a (18..288, E = 18: radix-6, 3 and 2 stages) and N = 27
+// (E = 27, one thread per pencil); other lengths use the generic Stockham
+// kernels.
 #pragma once
 
 namespace sca {
 namespace pencil {
 
+constexpr int odd_part(int n) {
+  int m = n;
+  while (m > 0 && m % 2 == 0) m /= 2;
+  return m;
+}
 constexpr bool supported(int n) {
   if (n < 8) return false;
-  int m = n;
-  while (m % 2 == 0) m /= 2;
+  const int m = odd_part(n);
   if (m == 1) return n <= 512;
-  return m == 3 && n >= 12 && n <= 384;
+  if (m == 3) return n >= 12 && n <= 384;
+  if (m == 9) return n >= 18 && n <= 288;
+  return n == 27;
 }
 constexpr int points_per_thread(int n) {
-  int m = n;
-  while (m % 2 == 0) m /= 2;
-  return m == 1 ? 8 : 12;
+  const int m = odd_part(n);
+  return m == 1 ? 8 : m == 3 ? 12 : m == 9 ? 18 : 27;
 }
 // single precision: 2^a pencils of 32 points or more hold 16 points per
 // thread, so that a thread moves as many bytes as in double precision

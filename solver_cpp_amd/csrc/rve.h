@@ -212,6 +212,12 @@ class Rve {
   int stress_mask[9];
   double rot_q[4];
   double delta_t = 0.0, last_dt = 0.0, last_div_sum = 0.0;
+  // Homogeneous fields, known on the host: X (F of the solver) after init,
+  // F_last_inc forwarded from it and the rate of an increment without a rate
+  // guess; their common 9 values.  forward() averages a homogeneous
+  // forwarded field the way the reference's sum does (see forward).
+  bool x_uniform = false, last_uniform = false, rate_uniform = false;
+  double x_value[9] = {}, last_value[9] = {}, rate_value[9] = {};
   int total_iter = 0;
   double err_BC = 0.0, err_div = 0.0;
   bool terminally_ill = false;

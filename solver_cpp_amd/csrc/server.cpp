@@ -206,6 +206,7 @@ class Server {
       o->batches = b.batches;
       o->batched_evaluations = b.evaluations;
       o->max_batch = b.max_batch;
+      o->batch_seconds = b.busy_seconds;
     }
   }
 
@@ -342,10 +343,23 @@ class Server {
           jobs_.erase(jid);
         }
       }
-      drop.reset();  // free outside the lock
+      drop.reset();  // release outside the lock
+      trim_cache();
       cv_done_.notify_all();
       cv_work_.notify_all();
     }
+  }
+
+  // Released contexts leave their device blocks in the library's cache
+  // (reused by the next jobs of the same sizes without hipMalloc); the cache
+  // is held to the part of the budget the resident jobs leave free.
+  void trim_cache() {
+    int64_t free_budget;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      free_budget = std::max<int64_t>(0, budget_ - used_);
+    }
+    if ((int64_t)pool_cached_bytes() > free_budget) pool_trim((size_t)free_budget);
   }
 
   // checkpoint an idle resident job to host memory and free its device
@@ -375,6 +389,7 @@ class Server {
       if (!v.pending.empty()) ready_.push_back(v.id);
     }
     drop.reset();
+    trim_cache();
     cv_work_.notify_all();
   }
 

@@ -263,8 +263,7 @@ KTimer::~KTimer() {
 
 // ---------------------------------------------------------------------------
 double* Rve::dalloc(size_t n) {
-  double* p = nullptr;
-  SCA_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(double)));
+  double* p = static_cast<double*>(dev_alloc(std::max<size_t>(n, 1) * sizeof(double)));
   allocs.push_back(p);
   return p;
 }
@@ -337,17 +336,17 @@ void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
   build_matconst(d.material, d.phase_num, mc);
   // non-blocking: contexts driven from several host threads never wait on
   // each other through the null stream
-  SCA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  st = stream_get();
   fft.init(cells);
   if (d.grid_num.fft_plan_mode == 1) fft.tune(9, st);
   ngm = new Ngmres(9 * N, st, &kt);
   ngm->after_sync = [this] { complete_response_scalars(); };
-  SCA_HIP(hipHostMalloc((void**)&h_scal, 16 * sizeof(double), hipHostMallocDefault));
+  h_scal = static_cast<double*>(host_alloc(16 * sizeof(double)));
   mixed = d.precision == SC_PRECISION_MIXED;
   if (mixed) {
     fft32.init(cells);
     if (d.grid_num.fft_plan_mode == 1) fft32.tune(9, st);
-    SCA_HIP(hipMalloc(&spec32, 2 * 9 * K * sizeof(float)));
+    spec32 = static_cast<float*>(dev_alloc(2 * 9 * K * sizeof(float)));
   }
   // material point planes
   mp.N = N;
@@ -365,8 +364,8 @@ void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
   mp.Fe = dalloc(9 * N);
   mp.xi = dalloc(12 * N);
   mp.gamma = dalloc(12 * N);
-  SCA_HIP(hipMalloc(&mp.iters, 2 * N * sizeof(int)));
-  SCA_HIP(hipMalloc(&mp.repivot, (N + 1) * sizeof(int)));
+  mp.iters = static_cast<int*>(dev_alloc(2 * N * sizeof(int)));
+  mp.repivot = static_cast<int*>(dev_alloc((N + 1) * sizeof(int)));
   sub = dalloc((size_t)mp_sub_planes() * N);
   X = dalloc(9 * N);
   F_last_inc = dalloc(9 * N);
@@ -387,7 +386,7 @@ void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
   d_n2 = dalloc(N);
   d_scr_partial = dalloc((size_t)2 * red_blocks());
   d_ill = reinterpret_cast<int*>(small + SM_ILL);
-  SCA_HIP(hipMalloc(&d_minmax_i, 2 * red_blocks() * sizeof(long long)));
+  d_minmax_i = static_cast<long long*>(dev_alloc(2 * red_blocks() * sizeof(long long)));
   if (d.grid_num.derivative != 0) {
     // xi table in the kernel layout: (3 complex) per k = i + nr*(y + Ny*z)
     std::vector<double> tab(6 * K);
@@ -436,7 +435,7 @@ void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
     SCA_HIP(hipMemcpyAsync(d_fp, fp.data(), fp.size() * sizeof(double), hipMemcpyHostToDevice, st));
     launch_mp_init(mp, d_fp, d.material.xi_0_sl, st);
     SCA_HIP(hipStreamSynchronize(st));
-    SCA_HIP(hipFree(d_fp));
+    dev_free(d_fp);
     allocs.pop_back();
   }
   std::memset(C_ref, 0, sizeof C_ref);
@@ -459,24 +458,26 @@ Rve::~Rve() { release(); }
 
 // frees whatever set-up allocated (also after a set-up that threw part way)
 void Rve::release() {
+  // the blocks go back to the library's caches: nothing may still use them
+  if (st) (void)hipStreamSynchronize(st);
   release_residual_graph();
   delete ngm;
   ngm = nullptr;
-  if (h_scal) (void)hipHostFree(h_scal);
+  host_free(h_scal);
   h_scal = nullptr;
-  if (spec32) (void)hipFree(spec32);
+  dev_free(spec32);
   spec32 = nullptr;
   fft32.release();
-  for (double* p : allocs) (void)hipFree(p);
+  for (double* p : allocs) dev_free(p);
   allocs.clear();
-  if (mp.iters) (void)hipFree(mp.iters);
-  if (mp.repivot) (void)hipFree(mp.repivot);
+  dev_free(mp.iters);
+  dev_free(mp.repivot);
   mp.iters = nullptr;
   mp.repivot = nullptr;
-  if (d_minmax_i) (void)hipFree(d_minmax_i);
+  dev_free(d_minmax_i);
   d_minmax_i = nullptr;
   fft.release();
-  if (st) (void)hipStreamDestroy(st);
+  stream_put(st);
   st = nullptr;
 }
 
@@ -824,15 +825,15 @@ int Rve::converged() const {
 // ---------------------------------------------------------------------------
 Ngmres::Ngmres(int64_t n_, hipStream_t st_, KTimer* kt_) : n(n_), st(st_), kt(kt_) {
   auto a = [&](double*& p, size_t m) {
-    SCA_HIP(hipMalloc(&p, std::max<size_t>(m, 1) * sizeof(double)));
+    p = static_cast<double*>(dev_alloc(std::max<size_t>(m, 1) * sizeof(double)));
     bufs.push_back(p);
   };
   a(partial, (size_t)red_blocks() * MAX_RED_VEC);
   a(small, MAX_RED_VEC);
 }
 Ngmres::~Ngmres() {
-  for (double* p : bufs) (void)hipFree(p);
-  for (double* p : pool) (void)hipFree(p);
+  for (double* p : bufs) dev_free(p);  // the owner synchronised the stream
+  for (double* p : pool) dev_free(p);
 }
 double* Ngmres::take() {
   auto used = [&](const double* p) {
@@ -845,8 +846,7 @@ double* Ngmres::take() {
   };
   for (double* p : pool)
     if (!used(p)) return p;
-  double* p = nullptr;
-  SCA_HIP(hipMalloc(&p, std::max<int64_t>(n, 1) * sizeof(double)));
+  double* p = static_cast<double*>(dev_alloc(std::max<int64_t>(n, 1) * sizeof(double)));
   pool.push_back(p);
   return p;
 }
@@ -1281,6 +1281,7 @@ void Rve::load(const std::vector<char>& blob) {
   }
   f_eval = nullptr;  // mp.F holds the restored F
   scal_pending = false;
+  x_uniform = last_uniform = rate_uniform = false;
   SCA_HIP(hipStreamSynchronize(st));
 }
 
@@ -1291,6 +1292,9 @@ void Rve::init() {
   SCA_HIP(hipMemcpyAsync(F_last_inc, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice, st));
   SCA_HIP(hipMemcpyAsync(X, eye.data(), 9 * N * sizeof(double), hipMemcpyHostToDevice, st));
   SCA_HIP(hipStreamSynchronize(st));
+  x_uniform = last_uniform = true;
+  rate_uniform = false;
+  for (int q = 0; q < 9; ++q) x_value[q] = last_value[q] = (q % 4 == 0) ? 1.0 : 0.0;
   update_coords();
   constitutive_response(X, 0.0);
   fetch_response_scalars(false);
@@ -1319,6 +1323,10 @@ void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_r
     SCA_HIP(hipMemcpyAsync(small + SM_AV, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
     launch_rate(guess ? 1 : 0, F_last_inc, X, dt_old, small + SM_AV, F_dot, N, st);
     SCA_HIP(hipMemcpyAsync(F_last_inc, X, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    rate_uniform = !guess;
+    std::memcpy(rate_value, rotated, sizeof rate_value);
+    last_uniform = x_uniform;
+    std::memcpy(last_value, x_value, sizeof last_value);
     sync_response_F();
     launch_mp_forward(mp, st);  // f_materialpoint_forward
   }
@@ -1334,14 +1342,36 @@ void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_r
   h::rot2_active(stp.rot_q, F_aim, rotated);  // load -> field frame
   SCA_HIP(hipMemcpyAsync(small + SM_AIM2, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
   launch_fwd_field(F_last_inc, F_dot, dt, X, 9 * N, st);
-  std::vector<int> ops(9, 2);
-  std::vector<const double*> a(9);
-  for (int q = 0; q < 9; ++q) a[q] = X + q * N;
-  launch_partial_reduce<double>(ops.data(), N, 9, a.data(), nullptr, partial, st);
-  launch_final_reduce(partial, 9, red_blocks(), small + SM_FSUM, st);
-  launch_fwd_diff(small + SM_FSUM, wgt, small + SM_AIM2, small + SM_FDIFF, st);
+  double diff[9];  // read by the copy below until the synchronisation
+  if (last_uniform && rate_uniform) {
+    // Every cell holds the same value v.  The reference averages the field
+    // with one add chain per component over the cells in storage order
+    // (mech_base.cpp:222, an Eigen sum over the cell dimensions); for N equal
+    // terms that chain's rounding errors all have one sign and its average
+    // is off by up to N/2 ulp (3.9e-12 for v = 1.0005 at 64^3).  The offset
+    // stays in the average of F through the whole increment and moves <P>
+    // of a first increment by about 1e-8 of itself, so here the sum is
+    // formed as that chain does, on the host (N adds per component, only
+    // for a homogeneous field: the first increment of a job).  Other
+    // fields are summed by the device's blocked reduction, whose error is
+    // of the order of log2(N) ulp.
+    double v[9], sum[9] = {};
+    for (int q = 0; q < 9; ++q) v[q] = last_value[q] + rate_value[q] * dt;
+    for (int64_t c = 0; c < N; ++c)
+      for (int q = 0; q < 9; ++q) sum[q] += v[q];
+    for (int q = 0; q < 9; ++q) diff[q] = sum[q] * wgt - rotated[q];
+    SCA_HIP(hipMemcpyAsync(small + SM_FDIFF, diff, sizeof diff, hipMemcpyHostToDevice, st));
+  } else {
+    std::vector<int> ops(9, 2);
+    std::vector<const double*> a(9);
+    for (int q = 0; q < 9; ++q) a[q] = X + q * N;
+    launch_partial_reduce<double>(ops.data(), N, 9, a.data(), nullptr, partial, st);
+    launch_final_reduce(partial, 9, red_blocks(), small + SM_FSUM, st);
+    launch_fwd_diff(small + SM_FSUM, wgt, small + SM_AIM2, small + SM_FDIFF, st);
+  }
   launch_sub_plane_const(X, small + SM_FDIFF, N, st);
   SCA_HIP(hipStreamSynchronize(st));
+  x_uniform = false;  // X now holds the forwarded field, changed by the solve
   for (int q = 0; q < 9; ++q) stress_mask[q] = stp.stress_mask[q];
   for (int q = 0; q < 4; ++q) rot_q[q] = stp.rot_q[q];
   delta_t = dt;

@@ -867,7 +867,9 @@ static FftPassDesc make_desc(int n, int sign) {
 // Pencil kernels for the supported lengths (compile-time N); false: use the
 // generic Stockham kernels.  bs: threads per block (pencils per block =
 // bs / threads per pencil); any valid value gives the same results.
-#define SCA_PENCIL_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(12) X(24) X(48) X(96) X(192) X(384)
+#define SCA_PENCIL_LENGTHS(X)                                                                          \
+  X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(12) X(24) X(48) X(96) X(192) X(384) X(18) X(36) X(72) X(144) \
+  X(288) X(27)
 
 constexpr size_t MAX_BLOCK_LDS = 64 * 1024;
 
@@ -981,7 +983,11 @@ template <typename T>
 int FftPlan<T>::shape(int axis) const {
   if (cfg[axis] > 0) return cfg[axis];
   const int n = cells[axis];
-  if (pencil_tp<T>(n) > 0) return 256;
+  if (pencil_tp<T>(n) > 0) {  // 256 threads, fewer where the pencils' LDS rows would not fit
+    int bs = 256;
+    while (bs > 64 && bs > pencil_tp<T>(n) && pencil_lds_of<T>(n, bs) > MAX_BLOCK_LDS) bs /= 2;
+    return bs;
+  }
   return axis == 0 ? x_rows_per_block(n) : pencils_per_block<T>(n);
 }
 
@@ -999,7 +1005,7 @@ void FftPlan<T>::init(const int cells_[3]) {
       h[2 * t] = (T)std::cos(ang);
       h[2 * t + 1] = (T)-std::sin(ang);
     }
-    SCA_HIP(hipMalloc(&tw[p], 2 * n * sizeof(T)));
+    tw[p] = static_cast<T*>(dev_alloc(2 * n * sizeof(T)));
     SCA_HIP(hipMemcpyAsync(tw[p], h.data(), 2 * n * sizeof(T), hipMemcpyHostToDevice, hipStreamPerThread));
     SCA_HIP(hipStreamSynchronize(hipStreamPerThread));
     make_desc(n, -1);  // validate
@@ -1009,7 +1015,7 @@ template <typename T>
 void FftPlan<T>::release() {
   for (int p = 0; p < 3; ++p)
     if (tw[p]) {
-      (void)hipFree(tw[p]);
+      dev_free(tw[p]);
       tw[p] = nullptr;
     }
 }
@@ -1117,8 +1123,8 @@ void FftPlan<T>::tune(int ncomp, hipStream_t st) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   std::array<int, 3> best = {0, 0, 0};
   try {
-    SCA_HIP(hipMalloc(&real, (size_t)ncomp * N * sizeof(double)));
-    SCA_HIP(hipMalloc(&spec, (size_t)2 * ncomp * K * sizeof(T)));
+    real = static_cast<double*>(dev_alloc((size_t)ncomp * N * sizeof(double)));
+    spec = static_cast<T*>(dev_alloc((size_t)2 * ncomp * K * sizeof(T)));
     SCA_HIP(hipMemsetAsync(real, 0, (size_t)ncomp * N * sizeof(double), st));
     SCA_HIP(hipMemsetAsync(spec, 0, (size_t)2 * ncomp * K * sizeof(T), st));
     SCA_HIP(hipEventCreate(&e0));
@@ -1161,14 +1167,15 @@ void FftPlan<T>::tune(int ncomp, hipStream_t st) {
   } catch (...) {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (real) (void)hipFree(real);
-    if (spec) (void)hipFree(spec);
+    (void)hipStreamSynchronize(st);
+    dev_free(real);
+    dev_free(spec);
     throw;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(real);
-  (void)hipFree(spec);
+  dev_free(real);  // the timing runs are complete (the last event was waited for)
+  dev_free(spec);
   for (int a = 0; a < 3; ++a) cfg[a] = best[a];
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[key] = best;
@@ -1448,19 +1455,19 @@ void launch_bc_aim(const double* sums, const double S[81], const double R[9], do
 }
 
 TableArena::~TableArena() {
-  if (hptr) (void)hipHostFree(hptr);
-  if (dptr) (void)hipFree(dptr);
+  host_free(hptr);
+  dev_free(dptr);
 }
 void TableArena::reserve(size_t bytes) {
   used = 0;
   if (bytes <= cap) return;
   bytes = std::max(bytes, 2 * cap);
-  if (hptr) SCA_HIP(hipHostFree(hptr));
-  if (dptr) SCA_HIP(hipFree(dptr));
+  host_free(hptr);  // the batch that used them has completed
+  dev_free(dptr);
   hptr = dptr = nullptr;
   cap = 0;
-  SCA_HIP(hipHostMalloc((void**)&hptr, bytes, hipHostMallocDefault));
-  SCA_HIP(hipMalloc((void**)&dptr, bytes));
+  hptr = static_cast<char*>(host_alloc(bytes));
+  dptr = static_cast<char*>(dev_alloc(bytes));
   cap = bytes;
 }
 

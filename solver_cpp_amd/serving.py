@@ -165,7 +165,8 @@ def run_batched(lib, specs, workers: int, budget: int, t: float = 5.0, N: int = 
     lat = np.array([r.t_end - r.t_start for rs in results for r in rs])
     return dict(wall=wall, updates=updates, value=updates / wall, failed=bad, max_concurrent=st.max_concurrent,
                 peak_bytes=st.peak_bytes, results=results, batches=st.batches, max_batch=st.max_batch,
-                batched_evaluations=st.batched_evaluations, increment_ms=lat * 1e3)
+                batched_evaluations=st.batched_evaluations, batch_seconds=st.batch_seconds,
+                increment_ms=lat * 1e3)
 
 
 def serving_requests(lib, n_req: int, rate: float, seed: int = 11, edge_range=(16, 128), precision: int = 0,
@@ -188,9 +189,9 @@ def serving_requests(lib, n_req: int, rate: float, seed: int = 11, edge_range=(1
     return reqs
 
 
-def run_serving(lib, reqs, workers: int, budget: int):
+def run_serving(lib, reqs, workers: int, budget: int, batched: bool = False, max_wait_ms: float = 2.0):
     """Submit each request at its arrival time; latency = end - arrival."""
-    srv = Server(lib, budget, workers)
+    srv = Server(lib, budget, workers, batched=batched, max_wait_ms=max_wait_ms)
     load = [tension_x(t=0.1, N=1)]
     t0 = time.perf_counter()
     ids = []
@@ -211,7 +212,7 @@ def run_serving(lib, reqs, workers: int, budget: int):
     upd = sum(o.material_point_updates for _, o in res)
     return dict(wall=wall, lat=lat, updates=upd, value=upd / wall, max_concurrent=st.max_concurrent,
                 peak_bytes=st.peak_bytes, failed=sum(1 for _, o in res if o.status != 0),
-                P_av=[np.array(o.P_av) for _, o in res])
+                P_av=[np.array(o.P_av) for _, o in res], batches=st.batches, max_batch=st.max_batch)
 
 
 def run_one_at_a_time(lib, reqs):
@@ -236,6 +237,9 @@ def main():
     ap.add_argument("--rves", type=int, default=256)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--rate", type=float, default=32.0, help="requests per second (serving)")
+    ap.add_argument("--rates", default="", help="serving: comma-separated rates to sweep (one JSON line each)")
+    ap.add_argument("--batched", action="store_true", help="joint launches of the running jobs' evaluations")
+    ap.add_argument("--max-wait-ms", type=float, default=2.0, help="batched: longest wait of a request for the others")
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
@@ -255,10 +259,11 @@ def main():
         lt, ln = args.load.split(",")
         specs = batched_specs(lib, args.rves, edges=edges, grains=(g_lo, g_hi), precision=prec, **num)
         kw = dict(t=float(lt), N=int(ln), increments=args.increments)
-        r = run_batched(lib, specs, args.workers, budget, **kw)
+        r = run_batched(lib, specs, args.workers, budget, batched=args.batched, max_wait_ms=args.max_wait_ms, **kw)
         print(f"batched: {args.workers} workers done in {r['wall']:.1f} s", file=sys.stderr, flush=True)
         seq = run_batched(lib, specs, 1, budget, **kw)
         print(json.dumps({"mode": "batched", "rves": args.rves, "workers": args.workers, "precision": args.precision,
+                          "joint_launches": args.batched,
                           "edges": list(edges), "grains": [g_lo, g_hi], "load": [float(lt), int(ln)],
                           "increments": args.increments or int(ln),
                           "mat_point_updates_per_s": r["value"], "wall_s": r["wall"],
@@ -266,21 +271,33 @@ def main():
                           "max_concurrent": r["max_concurrent"], "peak_gb": r["peak_bytes"] / 2 ** 30,
                           "failed_increments": r["failed"]}), flush=True)
     else:
-        reqs = serving_requests(lib, args.requests, args.rate, edge_range=(16, args.max_edge), precision=prec, **num)
-        r = run_serving(lib, reqs, args.workers, budget)
-        print(f"serving: {args.workers} workers done in {r['wall']:.1f} s", file=sys.stderr, flush=True)
-        seq = run_one_at_a_time(lib, reqs)
-        same = max(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(r["P_av"], seq["P_av"]))
-        print(json.dumps({"mode": "serving", "requests": args.requests, "rate": args.rate, "workers": args.workers,
-                          "precision": args.precision,
-                          "p50_ms": float(np.percentile(r["lat"], 50) * 1e3),
-                          "p99_ms": float(np.percentile(r["lat"], 99) * 1e3),
-                          "mat_point_updates_per_s": r["value"],
-                          "one_at_a_time": {"p50_ms": float(np.percentile(seq["lat"], 50) * 1e3),
-                                            "p99_ms": float(np.percentile(seq["lat"], 99) * 1e3),
-                                            "mat_point_updates_per_s": r["updates"] / seq["wall"]},
-                          "max_rel_diff_vs_alone": same, "max_concurrent": r["max_concurrent"],
-                          "peak_gb": r["peak_bytes"] / 2 ** 30, "failed": r["failed"]}), flush=True)
+        rates = [float(x) for x in args.rates.split(",")] if args.rates else [args.rate]
+        seq = None
+        for rate in rates:
+            reqs = serving_requests(lib, args.requests, rate, edge_range=(16, args.max_edge), precision=prec,
+                                    **num)
+            r = run_serving(lib, reqs, args.workers, budget, batched=args.batched, max_wait_ms=args.max_wait_ms)
+            print(f"serving at {rate} req/s: {args.workers} workers done in {r['wall']:.1f} s", file=sys.stderr,
+                  flush=True)
+            if seq is None:  # the same requests (the arrival times aside) at every rate
+                seq = run_one_at_a_time(lib, reqs)
+            serving_line(args, rate, r, seq)
+
+
+def serving_line(args, rate, r, seq):
+    """One JSON line of a serving run against the same requests one at a time."""
+    same = max(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(r["P_av"], seq["P_av"]))
+    print(json.dumps({"mode": "serving", "requests": args.requests, "rate": rate, "workers": args.workers,
+                      "joint_launches": args.batched, "batches": r["batches"], "max_batch": r["max_batch"],
+                      "precision": args.precision,
+                      "p50_ms": float(np.percentile(r["lat"], 50) * 1e3),
+                      "p99_ms": float(np.percentile(r["lat"], 99) * 1e3),
+                      "mat_point_updates_per_s": r["value"],
+                      "one_at_a_time": {"p50_ms": float(np.percentile(seq["lat"], 50) * 1e3),
+                                        "p99_ms": float(np.percentile(seq["lat"], 99) * 1e3),
+                                        "mat_point_updates_per_s": r["updates"] / seq["wall"]},
+                      "max_rel_diff_vs_alone": same, "max_concurrent": r["max_concurrent"],
+                      "peak_gb": r["peak_bytes"] / 2 ** 30, "failed": r["failed"]}), flush=True)
 
 
 if __name__ == "__main__":

@@ -84,12 +84,16 @@ def c2_pack(dump: str, extra_cells: str | None):
     N = c["edge"] ** 3
     cnt = np.load(os.path.join(dump, "counts.npz"))
     rng = np.random.default_rng(2024)
-    cells = set(rng.choice(N, 4096, replace=False).tolist())
+    rand = np.sort(rng.choice(N, 4096, replace=False))
+    extra = set()
     if extra_cells:
-        cells |= {int(x) for x in open(extra_cells).read().split()}
-    cells = np.array(sorted(cells), dtype=np.int64)
+        extra = {int(x) for x in open(extra_cells).read().split()} - set(rand.tolist())
+    cells = np.concatenate([rand, np.array(sorted(extra), dtype=np.int64)])
+    # the first n_random cells are the seeded random sample, the rest the
+    # listed ones (cells where rounding-level differences have moved a
+    # stopping decision of the material point)
     out = dict(iterations=cnt["iterations"], evaluations=cnt["evaluations"], P_av=cnt["P_av"], cells=cells,
-               sampled=np.array(C2_SAMPLED))
+               n_random=np.array(rand.size), sampled=np.array(C2_SAMPLED))
     for inc in C2_SAMPLED:
         d = np.load(os.path.join(dump, f"inc{inc}.npz"))
         for k in C2_FIELDS:

@@ -114,3 +114,72 @@ def test_c4_batched_against_oracle(lib):
             want = fx[f"{k}_{j}"]
             err = np.abs(got - want).max() / np.abs(want).max()
             assert err < 1e-10, f"rve {j} {k}: {err:.2e}"
+
+
+C5_FIXTURE = os.path.join(GOLDEN, "c5_fixture.npz")
+
+
+def test_serving_stream_alone_equals_mix(lib):
+    """The serving workload's first 24 requests (seed 11: Poisson arrivals at
+    64 per second, edges log-uniform in [16, 128] rounded to 2^a 3^b -- here
+    16, 18, 24, 27, 32, 48, 54, 64, 72, 81, 96, 108 and 128 -- one increment
+    of 1e-4 strain each) submitted at their arrival times to the batched
+    server, against each request run alone on an idle server: the same
+    iterations and evaluations and bitwise the same <P> and P.  Requests on
+    grids of at most 36^3 cells also match the oracle's runs
+    (c5_fixture.npz): identical counts, <P> and P on sampled cells within
+    1e-10."""
+    import time
+
+    from solver_cpp_amd.serving import serving_requests
+
+    reqs = serving_requests(lib, 24, rate=64.0, seed=11, edge_range=(16, 128), **EXAMPLE_NUMERICS)
+    print(f"[serving] edges {[s.cells[0] for _, s in reqs]}", flush=True)
+    load = [tension_x(t=0.1, N=1)]
+
+    def fields(srv, job, spec):
+        return srv.field(job, "P", spec.N)
+
+    alone = []
+    srv = Server(lib, 1 << 38, 1)
+    for _, spec in reqs:
+        job = srv.add_job(spec, load, keep_resident=True)
+        r = srv.wait(srv.submit(job), 300.0)
+        alone.append((r, fields(srv, job, spec)))
+        srv.remove_job(job)
+    srv.close()
+    srv = Server(lib, 1 << 38, 16, batched=True, max_wait_ms=5.0)
+    t0 = time.perf_counter()
+    ids = []
+    for arr, spec in reqs:
+        delay = arr - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        job = srv.add_job(spec, load, keep_resident=True)
+        ids.append((job, srv.submit(job)))
+    mix = []
+    for (job, rid), (_, spec) in zip(ids, reqs):
+        r = srv.wait(rid, 300.0)
+        mix.append((r, fields(srv, job, spec)))
+    st = srv.stats()
+    for job, _ in ids:
+        srv.remove_job(job)
+    srv.close()
+    assert st.max_concurrent >= 2 and st.max_batch >= 2, (st.max_concurrent, st.max_batch)
+    for (a, Pa), (m, Pm) in zip(alone, mix):
+        assert a.status == 0 and m.status == 0
+        assert (a.iterations, a.evaluations) == (m.iterations, m.evaluations)
+        np.testing.assert_array_equal(np.array(a.P_av), np.array(m.P_av))
+        np.testing.assert_array_equal(Pa, Pm)
+    if not os.path.exists(C5_FIXTURE):
+        pytest.skip("c5 fixture not generated")
+    fx = np.load(C5_FIXTURE)
+    for i in fx["index"]:
+        (r, P), spec = mix[i], reqs[i][1]
+        assert (r.iterations, r.evaluations) == (int(fx[f"iterations_{i}"]), int(fx[f"evaluations_{i}"])), i
+        ref = fx[f"P_av_{i}"]
+        assert np.abs(np.array(r.P_av) - ref).max() / np.abs(ref).max() < 1e-10, i
+        cells = fx[f"cells_{i}"]
+        want = fx[f"P_{i}"]
+        err = np.abs(P.reshape(spec.N, 9)[cells] - want).max() / np.abs(want).max()
+        assert err < 1e-10, (i, err)

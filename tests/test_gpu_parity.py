@@ -5,6 +5,7 @@ fields agree within 1e-10 relative to the field's magnitude, the bar the
 plain GPU form has to meet against the reference solver.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -31,10 +32,10 @@ def test_backend(product):
 
 # ---- spectral stages ---------------------------------------------------------------
 # generic lengths, and the compile-time pencil lengths (2^a from 8, 3 * 2^a from
-# 12) in every axis, with odd and even numbers of x rows
+# 12, 9 * 2^a from 18, and 27) in every axis, with odd and even numbers of x rows
 FFT_CELLS = [(2, 1, 1), (4, 3, 5), (8, 8, 8), (6, 10, 7), (16, 12, 9), (32, 32, 32), (20, 14, 22), (8, 3, 3),
              (12, 24, 16), (128, 8, 6), (24, 48, 12), (96, 2, 64), (256, 2, 3), (512, 1, 1), (384, 3, 1),
-             (192, 2, 12)]
+             (192, 2, 12), (18, 36, 27), (72, 27, 18), (144, 3, 36), (288, 2, 1), (27, 8, 72), (54, 18, 9)]
 
 
 @pytest.mark.parametrize("cells", FFT_CELLS)
@@ -198,6 +199,13 @@ def test_mp_update_other_exponents(product, oracle, n_sl, a_sl):
     assert same.mean() >= 0.99, same.mean()
     for k in ["P", "Fp", "Lp", "S", "Fe", "xi", "gamma", "dPdF"]:
         close(a[k][same], b[k][same], REL, k)
+    # points whose counts differ: within the stopping tests' own spread
+    if (~same).any():
+        for k in ("P", "Fp", "xi"):
+            v = float(np.abs(a[k][~same] - b[k][~same]).max() / np.abs(b[k]).max())
+            print(f"exponents {n_sl}/{a_sl}: {(~same).sum()} points with other counts, {k} spread {v:.2e}",
+                  flush=True)
+            assert v <= 5e-5, (k, v)
 
 
 def test_mp_update_substepping(product, oracle):
@@ -206,17 +214,29 @@ def test_mp_update_substepping(product, oracle):
     to their thresholds for some points, so rounding decides between two
     valid results: two CPU builds of the oracle (with and without fused
     multiply-add) already disagree on 201 of these 4096 points.  Required:
-    the same broken flags, at most 10 % of points with other iteration
-    counts, and the rest within 1e-10."""
+    the same broken flags, at most 300 points with other iteration counts,
+    the rest within 1e-10, and the points with other counts within twice the
+    spread of those two oracle builds (measured on the same batch: P 1.41e-3,
+    Fp 1.40e-4, xi 6.66e-4 of each field's maximum): a flipped stopping
+    decision changes the substep sequence, and with it the time
+    discretisation of the step, by that much and no more."""
     F, Fp0 = _random_batch(oracle, 2e-2)
     a = _mp_batch(product, F, Fp0, 0.5)
     b = _mp_batch(oracle, F, Fp0, 0.5)
     np.testing.assert_array_equal(a["broken"], b["broken"])
     same = (a["iterations"] == b["iterations"]).all(axis=1)
-    assert same.mean() >= 0.9, same.mean()
+    assert (~same).sum() <= 300, (~same).sum()
     assert (b["iterations"][:, 1] > 12).any()  # substepping happened
     for k in ["P", "Fp", "Lp", "xi", "gamma"]:
         close(a[k][same], b[k][same], REL, k)
+    diff = ~same & (b["broken"] == 0)
+    spread = {k: float(np.abs(a[k][diff] - b[k][diff]).max() / np.abs(b[k]).max()) if diff.any() else 0.0
+              for k in ("P", "Fp", "xi")}
+    print(f"substepping: {diff.sum()} of {len(same)} points with other counts, spread {spread}", flush=True)
+    bound = {"P": 2 * 1.41e-3, "Fp": 2 * 1.40e-4, "xi": 2 * 6.66e-4}
+    for k, v in spread.items():
+        assert v <= bound[k], (k, v)
+
 
 
 # ---- outer solver ----------------------------------------------------------------------
@@ -235,6 +255,29 @@ def test_ngmres_replays_fortran_trace(product, trace):
 def _pair(product, oracle, cells, grains, seed=1, ori_seed=None, **kw):
     mat, ori, size = polycrystal(cells, grains, seed=seed, ori_seed=ori_seed)
     return Rve(product, cells, size, mat, ori, **kw), Rve(oracle, cells, size, mat, ori, **kw)
+
+
+def test_first_increment_homogeneous_forward(product, oracle):
+    """The first increment forwards a homogeneous field, whose average the
+    reference takes with one add chain per component (its rounding error,
+    all of one sign, is 8.1e-13 at 32^3 for F_xx = 1.0005).  The forwarded F
+    must be the reference's to the last bits (not only to 1e-10): that
+    offset stays in the average of F through the increment, and <P> of the
+    first increment, small against the stiffness, moves by about 1e-8 of
+    itself with it.  Checked: F after forward within 1e-15, then the whole
+    first increment with identical counts and <P> within 1e-10."""
+    cells = (32, 32, 32)
+    a, b = _pair(product, oracle, cells, 24, seed=7, ori_seed=8, **EXAMPLE_NUMERICS)
+    a.init()
+    b.init()
+    load = tension_x(t=10.0, N=20)
+    a.forward(False, False, 0.5, 1.0, 10.0, load)
+    b.forward(False, False, 0.5, 1.0, 10.0, load)
+    np.testing.assert_allclose(a.field("F"), b.field("F"), rtol=0, atol=1e-15)
+    sa, sb = a.calculate_solution(), b.calculate_solution()
+    assert (sa.required_iterations, sa.residual_evaluations) == (sb.required_iterations, sb.residual_evaluations)
+    close(np.array(a.scalars().P_av), np.array(b.scalars().P_av), REL, "<P>")
+    close(a.field("P"), b.field("P"), REL, "P")
 
 
 @pytest.mark.parametrize("cells", [(8, 6, 4), (8, 6, 16), (7, 5, 12), (9, 8, 32), (6, 4, 64)])
@@ -286,6 +329,28 @@ def test_loadcase_16cubed_20grains(product, oracle):
     close(a.field("xi"), b.field("xi"), REL, "xi")
     close(a.scalars().P_av, b.scalars().P_av, REL, "P_av")
     close(a.field("x_p"), b.field("x_p"), 1e-12, "x_p")
+
+
+def test_loadcase_update_gamma(product, oracle):
+    """update_gamma = 1 (MechSolverBasic::calculate_solution,
+    mech_solver_basic.cpp:127 -> MechBase::update_gamma, mech_base.cpp:156-211):
+    the reference stiffness of the Gamma operator is taken again from
+    C_minMaxAvg before every solve instead of once at init.  12^3 grid, 8
+    grains, 6 tension increments: identical iterations and evaluations, F,
+    P and the final C_ref within 1e-10 of the oracle's."""
+    cells = (12, 12, 12)
+    a, b = _pair(product, oracle, cells, 8, seed=13, ori_seed=14, update_gamma=1, **EXAMPLE_NUMERICS)
+    a.init()
+    b.init()
+    load = [tension_x(t=3.0, N=6)]
+    ia, ea = a.loop_over_loadcase(load)
+    ib, eb = b.loop_over_loadcase(load)
+    np.testing.assert_array_equal(ia, ib)
+    np.testing.assert_array_equal(ea, eb)
+    close(a.field("F"), b.field("F"), REL, "F")
+    close(a.field("P"), b.field("P"), REL, "P")
+    sa, sb = a.scalars(), b.scalars()
+    close(sa.C_ref, sb.C_ref, 1e-12, "C_ref")
 
 
 def test_loadcase_64cubed_200grains_first_increments(product, oracle):
@@ -435,3 +500,51 @@ def test_mixed_precision_reuses_the_last_update(product):
     st = r.stats()
     assert st.residual_evaluations - st.fp32_evaluations >= n_inc
     assert st.material_point_updates < st.residual_evaluations * r.N
+
+
+C2_FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_fixture.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(C2_FIXTURE), reason="c2 fixture not generated")
+def test_loadcase_64cubed_200grains_all_increments():
+    """The drop-in configuration through all 40 increments (64^3 grid, 200
+    Voronoi grains, tension to 2 %) against the oracle's run of the same
+    load case (tests/golden/c2_fixture.npz, made by make_load_fixtures.py):
+    identical iterations and evaluations in every increment and <P> within
+    1e-10 relative in every increment; at increments 2, 10, 20 and 40, F, Fp
+    and xi within 1e-10 of their maxima on the sampled cells and P within
+    1e-10 on at least 99.8 % of the seeded random sample, every sampled cell
+    (those of the cluster included) within 5e-9.  Past the first increments
+    a rounding-level difference moves a stopping decision of the Lp Newton or
+    the state fixed point in a few cells, whose neighbours follow through
+    equilibrium: two CPU builds of the oracle (with and without fused
+    multiply-add) differ from each other by the same amount in the same
+    cells."""
+    from solver_cpp_amd.capi import Library
+    from conftest import PRODUCT_SO
+
+    fx = np.load(C2_FIXTURE)
+    product = Library(PRODUCT_SO)
+    cfg = CONFIGS["C2"]
+    e = cfg["edge"]
+    mat, ori, size = polycrystal((e, e, e), cfg["grains"], seed=cfg["seed"], ori_seed=cfg["ori_seed"])
+    a = Rve(product, (e, e, e), size, mat, ori, **EXAMPLE_NUMERICS)
+    a.init()
+    cells, n_rand = fx["cells"], int(fx["n_random"])
+    sampled = set(int(i) for i in fx["sampled"])
+    N = e ** 3
+    for k, inc in enumerate(a.increments([tension_x(t=cfg["t"], N=cfg["N"])])):
+        assert (inc["iterations"], inc["evaluations"]) == (fx["iterations"][k], fx["evaluations"][k]), k + 1
+        close(np.array(a.scalars().P_av), fx["P_av"][k], REL, f"<P> increment {k + 1}")
+        if inc["increment"] in sampled:
+            i = inc["increment"]
+            for name in ("F", "Fp", "xi"):
+                close(a.field(name).reshape(N, -1)[cells], fx[f"{name}_{i}"], REL, f"{name} increment {i}")
+            P = a.field("P").reshape(N, 9)[cells]
+            ref = fx[f"P_{i}"]
+            err = np.abs(P - ref).max(axis=1) / np.abs(ref).max()
+            frac = float((err[:n_rand] <= REL).mean())
+            print(f"64^3 increment {i}: P within 1e-10 on {frac:.4f} of the random sample, max {err.max():.2e}",
+                  flush=True)
+            assert frac >= 0.998, (i, frac)
+            assert err.max() <= 5e-9, (i, err.max())

@@ -149,6 +149,45 @@ def test_checkpoint_restart_continues_bitwise(lib, tmp_path):
 
 
 @pytest.mark.gpu
+def test_restored_run_matches_uninterrupted_oracle(lib, tmp_path):
+    """A context checkpointed after three increments of the tension case,
+    restored into a new context and driven through two more increments
+    matches the oracle's uninterrupted five increments: identical iterations
+    and evaluations, F, P, Fp and xi within 1e-10 (the restart of
+    phase_mechanical.f90:1264-1298 and the grid solver's F, F_lastInc)."""
+    from conftest import ORACLE_SO
+
+    oracle = Library(ORACLE_SO)
+    mat, ori, size = polycrystal((12, 12, 12), 6, seed=3)
+    load = [tension_x(t=2.5, N=5)]
+    a = Rve(lib, (12, 12, 12), size, mat, ori, itmin=4, itmax=40)
+    a.init()
+    gen = a.increments(load)
+    first = [next(gen) for _ in range(3)]
+    path = str(tmp_path / "rve.ckpt")
+    a.checkpoint(path)
+    a.close()
+    b = Rve(lib, (12, 12, 12), size, mat, ori, itmin=4, itmax=40)
+    b.restore(path)
+    rest = []
+    for _ in range(2):  # increments 4 and 5, driven as increments() drives them
+        b.forward(False, True, 0.5, 0.5, 2.5 - 0.5 * (3 + len(rest)), load[0])
+        sol = b.calculate_solution()
+        assert sol.converged
+        b.update_coords()
+        rest.append((sol.required_iterations, sol.residual_evaluations))
+    o = Rve(oracle, (12, 12, 12), size, mat, ori, itmin=4, itmax=40)
+    o.init()
+    ref = [(d["iterations"], d["evaluations"]) for d in o.increments(load)]
+    assert [(d["iterations"], d["evaluations"]) for d in first] + rest == ref
+    for k in ("F", "P", "Fp", "xi"):
+        x, y = b.field(k), o.field(k)
+        assert np.abs(x - y).max() / np.abs(y).max() <= 1e-10, k
+    b.close()
+    o.close()
+
+
+@pytest.mark.gpu
 def test_eviction_to_host_changes_no_result(lib):
     """Four kept-resident jobs advanced in rounds with HBM for two: idle jobs
     are checkpointed to host memory and restored on their next request, and
