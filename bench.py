@@ -51,9 +51,11 @@ FP64_PEAK_TFLOPS = 78.6
 # flops per Newton iteration (Fe, Hooke, 12 slip rates, residual, traceless
 # 8x8 Jacobian assembly 1610, LU 470), per state iteration and per cell, and
 # bytes of state read and written per cell and evaluation (F F0 Fp0 Lp S0
-# xi0 gamma0 in, Lp S P Fp Fe xi gamma out, 2 counters).  The executed FP64
-# instructions counted by rocprofv3 on the same command are recorded next to
-# these in profiles/.
+# xi0 gamma0 in, Lp S P Fp Fe xi gamma out, 2 counters).  The FP64
+# instructions rocprofv3 counts over the timed launches of this workload
+# (x 64 lanes x the active-lane share) give fewer flops than this count
+# (the compiler folds the slip systems' 0 / +-1 coefficients); the ratio is
+# recorded in profiles/ and scales the count.
 RESPONSE_FLOPS_NEWTON = 2800
 RESPONSE_FLOPS_STATE = 850
 RESPONSE_FLOPS_CELL = 300
@@ -63,19 +65,25 @@ RESPONSE_BYTES_PER_CELL = (9 * 5 + 12 * 2 + 9 * 5 + 12 * 2) * 8 + 8
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this script and scaled by the
 # same counters on a copy kernel of known size; recorded in profiles/
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_128cubed_1000grains.md")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "response_kernel_counters_128cubed_1000grains.md")
 
 
-def recorded_traffic(kernel: str):
-    """Bytes per launch recorded for `kernel` ('<kernel>_bytes_per_launch: N'), or None."""
+def recorded(path: str, key: str):
+    """The value of a '`key: N`' line of a profile summary, or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
-                key, _, val = line.strip().strip("`").partition(":")
-                if key.strip() == f"{kernel}_bytes_per_launch":
+                k, _, val = line.strip().strip("`").partition(":")
+                if k.strip() == key:
                     return float(val)
     except OSError:
         return None
     return None
+
+
+def recorded_traffic(kernel: str):
+    """HBM bytes per launch recorded for `kernel`, or None."""
+    return recorded(TRAFFIC_FILE, f"{kernel}_bytes_per_launch")
 
 
 def log(msg: str):
@@ -130,15 +138,19 @@ def roofline(res):
     launches = max(d["launches"], 1)
     avg_s = d["ms"] / launches * 1e-3
     cells = d["points"] / launches
-    flops = (st.newton_iterations * RESPONSE_FLOPS_NEWTON + st.state_iterations * RESPONSE_FLOPS_STATE +
-             d["points"] * RESPONSE_FLOPS_CELL) / launches
+    by_count = (st.newton_iterations * RESPONSE_FLOPS_NEWTON + st.state_iterations * RESPONSE_FLOPS_STATE +
+                d["points"] * RESPONSE_FLOPS_CELL) / launches
+    calib = recorded(COUNTERS_FILE, "response_flops_counter_over_count") or 1.0
+    flops = by_count * calib
     ach = flops / avg_s / 1e12
     return {"kernel": "response", "launches": d["launches"], "avg_ms": d["ms"] / launches,
             "bound": "fp64-valu", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / FP64_PEAK_TFLOPS,
             "note": "FP64 vector ALU work (no dense contraction, no MFMA): peak is MI355X's FP64 vector "
-                    "78.6 TFLOP/s; flops by the kernel's own Newton/state iteration counts",
-            "flops_per_launch": flops, "cells_per_launch": cells,
+                    "78.6 TFLOP/s; flops: the kernel's own Newton/state iteration counts x per-iteration "
+                    "counts, scaled by the executed-FP64-instruction ratio rocprofv3 recorded for this workload",
+            "flops_per_launch": flops, "flops_per_launch_by_count": by_count, "flop_calibration": calib,
+            "cells_per_launch": cells,
             "hbm_gbs_algorithmic": RESPONSE_BYTES_PER_CELL * cells / avg_s / 1e9,
             "traffic": recorded_traffic("response")}
 
@@ -241,7 +253,7 @@ def _oracle_sample(lib, c, grains, args, loads, n_inc):
     return c ** 3 * evals / wall, evals, wall
 
 
-def cpu_baseline(args, loads):
+def cpu_baseline(args, loads, timed_evals: int, gpu_wall: float):
     """The CPU oracle (the FP64 C++ restatement, OpenMP over material points)
     on bounded samples of the same workload, with the same grain size (grains
     scaled by the volume) through the first increments of the same load case:
@@ -272,7 +284,7 @@ def cpu_baseline(args, loads):
                                                     f"{args.cpu_sample_increments} increments ({e_one} evaluations, "
                                                     f"{w_one:.1f} s)"},
            "cpu_model": model, "nproc": nproc}
-    if args.cpu_128:
+    if not args.no_cpu_128:
         c = args.cells
         mat, ori, size = polycrystal((c, c, c), args.grains, seed=args.seed, ori_seed=args.ori_seed)
         rve = Rve(lib, (c, c, c), size, mat, ori, **EXAMPLE_NUMERICS)
@@ -283,10 +295,12 @@ def cpu_baseline(args, loads):
         rve.form_residual(F)
         t_eval = time.perf_counter() - t0
         rve.close()
-        out["c3_evaluation"] = {"seconds": t_eval, "updates_per_s": c ** 3 / t_eval,
-                                "extrapolation": "full 60-increment run = its residual evaluations (GPU count) x "
-                                                 "this time"}
-        log(f"[bench] cpu oracle one 128^3 evaluation: {t_eval:.1f} s")
+        out["workload_evaluation"] = {
+            "cells": [c, c, c], "seconds": t_eval, "updates_per_s": c ** 3 / t_eval, "threads": threads,
+            "timed_increments_extrapolated_s": timed_evals * t_eval, "gpu_plain_fp64_timed_s": gpu_wall,
+            "extrapolation": f"the timed increments' {timed_evals} residual evaluations (plain FP64 form, the GPU's "
+                             f"count; the oracle takes the same iterations) x this evaluation's time"}
+        log(f"[bench] cpu oracle one {c}^3 evaluation: {t_eval:.1f} s")
     return out
 
 
@@ -306,7 +320,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-cells", type=int, default=32)
     ap.add_argument("--cpu-sample-increments", type=int, default=2)
-    ap.add_argument("--cpu-128", action="store_true", help="also time one 128^3 residual evaluation on the CPU")
+    ap.add_argument("--no-cpu-128", action="store_true", help="skip the CPU time of one residual evaluation of the "
+                                                                 "workload itself")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--batched-rves", type=int, default=256)
     ap.add_argument("--no-tight", action="store_true", help="skip the tight-tolerance precision comparison")
@@ -331,6 +346,7 @@ def main():
     res = plain if args.precision == "fp64" else run_gpu(args, loads, mat, ori, size, args.precision)
     value = res["updates"] / res["wall"]
     plain_value = plain["updates"] / plain["wall"]
+    plain_evals, plain_wall = int(plain["evals"]), float(plain["wall"])
     lat_ms = np.array(res["lat"]) * 1e3
     plain_ms = np.array(plain["lat"]) * 1e3
     line = {
@@ -407,7 +423,7 @@ def main():
     if not args.no_serving:
         line["serving"] = run_serving_sweep(args)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args, loads)
+        line["cpu_baseline"] = cpu_baseline(args, loads, plain_evals, plain_wall)
     print(json.dumps(line), flush=True)
 
 

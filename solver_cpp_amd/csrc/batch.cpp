@@ -219,16 +219,18 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
         div[j] = BatchReduce{v.partial, gamma_blocks(K), 1, v.small + SM_DIV};
       }
       launch_copy_batch(gather.data(), m, ta_, st_);
+      // the x passes pair rows within each RVE's 9 planes (set_comp 9): the
+      // transforms give the bits of the RVE transformed alone
       if (g.fp32) {
-        g.f32.forward(g.real, static_cast<float*>(g.spec), 9 * m, st_);
+        g.f32.forward(g.real, static_cast<float*>(g.spec), 9 * m, st_, 9);
         launch_gamma_div_batch<float>(gam.data(), m, K, ta_, st_);
       } else {
-        g.f64.forward(g.real, static_cast<double*>(g.spec), 9 * m, st_);
+        g.f64.forward(g.real, static_cast<double*>(g.spec), 9 * m, st_, 9);
         launch_gamma_div_batch<double>(gam.data(), m, K, ta_, st_);
       }
       launch_final_reduce_batch(div.data(), m, ta_, st_);
-      if (g.fp32) g.f32.backward(static_cast<float*>(g.spec), g.real, 9 * m, 1.0, st_);
-      else g.f64.backward(static_cast<double*>(g.spec), g.real, 9 * m, 1.0, st_);
+      if (g.fp32) g.f32.backward(static_cast<float*>(g.spec), g.real, 9 * m, 1.0, st_, 9);
+      else g.f64.backward(static_cast<double*>(g.spec), g.real, 9 * m, 1.0, st_, 9);
       launch_copy_batch(scatter.data(), m, ta_, st_);
     }
   }

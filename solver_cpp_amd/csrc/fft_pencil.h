@@ -235,12 +235,25 @@ __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) strided_kernel(cpx<T>* data,
   }
 }
 
+// Rows of the pencil that pairs rows ra and rb of a set of set_rows rows
+// (sets of rows are transformed independently: a set's last row of an odd
+// count is paired with zeros, never with the next set's first row).  An
+// absent row gets the index nrows.
+__device__ __forceinline__ void pencil_rows(int64_t pencil, int64_t set_rows, int64_t nrows, int64_t& ra,
+                                            int64_t& rb) {
+  const int64_t pps = (set_rows + 1) / 2;  // pencils per set
+  const int64_t set = pencil / pps, q = pencil - set * pps;
+  ra = set * set_rows + 2 * q;
+  rb = 2 * q + 1 < set_rows ? ra + 1 : nrows;
+  if (ra >= nrows) ra = rb = nrows;
+}
+
 // x pass, real -> half complex, two real rows per complex pencil
 // (z = x_a + i x_b): X_a[k] = (Z[k] + conj Z[N-k]) / 2,
 // X_b[k] = (Z[k] - conj Z[N-k]) / (2 i)
 template <typename T, typename TI, int N>
 __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, cpx<T>* out, int64_t nrows,
-                                                                 const cpx<T>* tw_g) {
+                                                                 int64_t set_rows, const cpx<T>* tw_g) {
   using S = Shape<T, N>;
   constexpr int NR = N / 2 + 1;
   extern __shared__ unsigned char smem_raw[];
@@ -248,7 +261,8 @@ __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, c
   const int P = blockDim.x / S::TP;
   const cpx<T>* tw = stage_tw<T, N>(lds + P * S::LD, tw_g);
   const int t = threadIdx.x % S::TP, p = threadIdx.x / S::TP;
-  const int64_t ra = 2 * ((int64_t)blockIdx.x * P + p), rb = ra + 1;
+  int64_t ra, rb;
+  pencil_rows((int64_t)blockIdx.x * P + p, set_rows, nrows, ra, rb);
   cpx<T>* row = lds + p * S::LD;
   cpx<T> v[S::E];
 #pragma unroll
@@ -274,7 +288,7 @@ __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, c
 // the real modes k = 0 and k = N/2 are dropped), scaled; two rows per pencil
 template <typename T, typename TO, int N>
 __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_c2r_kernel(const cpx<T>* in, TO* out, int64_t nrows,
-                                                                 const cpx<T>* tw_g, T scale) {
+                                                                 int64_t set_rows, const cpx<T>* tw_g, T scale) {
   using S = Shape<T, N>;
   constexpr int NR = N / 2 + 1;
   extern __shared__ unsigned char smem_raw[];
@@ -282,7 +296,8 @@ __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_c2r_kernel(const cpx<T>* i
   const int P = blockDim.x / S::TP;
   const cpx<T>* tw = stage_tw<T, N>(lds + P * S::LD, tw_g);
   const int t = threadIdx.x % S::TP, p = threadIdx.x / S::TP;
-  const int64_t ra = 2 * ((int64_t)blockIdx.x * P + p), rb = ra + 1;
+  int64_t ra, rb;
+  pencil_rows((int64_t)blockIdx.x * P + p, set_rows, nrows, ra, rb);
   cpx<T> v[S::E];
 #pragma unroll
   for (int j = 0; j < S::E; ++j) {

@@ -27,16 +27,21 @@ struct FftPlan {
   // the fastest (measured once per precision, grid and ncomp per process)
   void tune(int ncomp, hipStream_t st);
   // real: ncomp planes of N (double or T, converted in the x pass); spec:
-  // ncomp planes of K complex T (interleaved re, im)
+  // ncomp planes of K complex T (interleaved re, im).  set_comp (0: ncomp)
+  // divides the components into fields of that many: the x pass pairs two
+  // real rows per complex pencil only within a field, so a field's result
+  // does not depend on the fields transformed with it.
   template <typename TI>
-  void forward(const TI* real, T* spec, int ncomp, hipStream_t st) const;
+  void forward(const TI* real, T* spec, int ncomp, hipStream_t st, int set_comp = 0) const;
   template <typename TO>
-  void backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st) const;  // spec is clobbered
+  void backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st,
+                int set_comp = 0) const;  // spec is clobbered
   void strided_passes(void* spec, int ncomp, int sign, hipStream_t st) const;
 
  private:
   int shape(int axis) const;
-  void x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp, hipStream_t st) const;
+  void x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp, hipStream_t st,
+                  int set_comp) const;
   void strided_pass(cpx<T>* S, int ncomp, int sign, int axis, int shp, hipStream_t st) const;
 };
 

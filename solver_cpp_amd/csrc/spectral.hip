@@ -911,15 +911,17 @@ static std::vector<int> pencil_block_sizes(int n) {
   return out;
 }
 
+// rows: set_rows rows per independently paired set (rows is a multiple)
 template <typename T, typename TI>
-static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, const cpx<T>* tw, int bs, hipStream_t st) {
+static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, int64_t set_rows, const cpx<T>* tw,
+                         int bs, hipStream_t st) {
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
     using S = pencil::Shape<T, N>;                                                                              \
-    const int64_t pencils = (rows + 1) / 2, P = bs / S::TP;                                                  \
+    const int64_t pencils = rows / set_rows * ((set_rows + 1) / 2), P = bs / S::TP;                          \
     hipLaunchKernelGGL((pencil::x_r2c_kernel<T, TI, N>), dim3((unsigned)((pencils + P - 1) / P)), dim3(bs),   \
-                       (pencil_lds<T, N>(bs)), st, in, out, rows, tw);                                       \
+                       (pencil_lds<T, N>(bs)), st, in, out, rows, set_rows, tw);                             \
     return true;                                                                                             \
   }
     SCA_PENCIL_LENGTHS(SCA_CASE)
@@ -929,15 +931,15 @@ static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, const c
 }
 
 template <typename T, typename TO>
-static bool pencil_x_c2r(int n, const cpx<T>* in, TO* out, int64_t rows, const cpx<T>* tw, T scale, int bs,
-                         hipStream_t st) {
+static bool pencil_x_c2r(int n, const cpx<T>* in, TO* out, int64_t rows, int64_t set_rows, const cpx<T>* tw,
+                         T scale, int bs, hipStream_t st) {
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
     using S = pencil::Shape<T, N>;                                                                              \
-    const int64_t pencils = (rows + 1) / 2, P = bs / S::TP;                                                  \
+    const int64_t pencils = rows / set_rows * ((set_rows + 1) / 2), P = bs / S::TP;                          \
     hipLaunchKernelGGL((pencil::x_c2r_kernel<T, TO, N>), dim3((unsigned)((pencils + P - 1) / P)), dim3(bs),   \
-                       (pencil_lds<T, N>(bs)), st, in, out, rows, tw, scale);                                \
+                       (pencil_lds<T, N>(bs)), st, in, out, rows, set_rows, tw, scale);                      \
     return true;                                                                                             \
   }
     SCA_PENCIL_LENGTHS(SCA_CASE)
@@ -1021,14 +1023,17 @@ void FftPlan<T>::release() {
 }
 
 template <typename T>
-void FftPlan<T>::x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp,
-                            hipStream_t st) const {
+void FftPlan<T>::x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp, hipStream_t st,
+                            int set_comp) const {
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   const int64_t rows = (int64_t)ncomp * Ny * Nz;
+  const int64_t set_rows = (int64_t)(set_comp > 0 ? set_comp : ncomp) * Ny * Nz;
+  if (rows % set_rows != 0) throw std::runtime_error("FFT: components not a multiple of the field size");
   const cpx<T>* t0 = reinterpret_cast<const cpx<T>*>(tw[0]);
   bool done;
-  if (real_is_double) done = pencil_x_r2c<T, double>(Nx, static_cast<const double*>(real), S, rows, t0, shp, st);
-  else done = pencil_x_r2c<T, T>(Nx, static_cast<const T*>(real), S, rows, t0, shp, st);
+  if (real_is_double)
+    done = pencil_x_r2c<T, double>(Nx, static_cast<const double*>(real), S, rows, set_rows, t0, shp, st);
+  else done = pencil_x_r2c<T, T>(Nx, static_cast<const T*>(real), S, rows, set_rows, t0, shp, st);
   if (!done) {
     FftPassDesc d = make_desc(Nx, -1);
     const size_t sh = 2 * (size_t)shp * Nx * sizeof(cpx<T>);
@@ -1044,9 +1049,9 @@ void FftPlan<T>::x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, in
 
 template <typename T>
 template <typename TI>
-void FftPlan<T>::forward(const TI* real, T* spec, int ncomp, hipStream_t st) const {
+void FftPlan<T>::forward(const TI* real, T* spec, int ncomp, hipStream_t st, int set_comp) const {
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
-  x_pass_fwd(real, sizeof(TI) == sizeof(double), S, ncomp, shape(0), st);
+  x_pass_fwd(real, sizeof(TI) == sizeof(double), S, ncomp, shape(0), st, set_comp);
   strided_passes(S, ncomp, -1, st);
 }
 
@@ -1078,13 +1083,15 @@ void FftPlan<T>::strided_passes(void* Sv, int ncomp, int sign, hipStream_t st) c
 
 template <typename T>
 template <typename TO>
-void FftPlan<T>::backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st) const {
+void FftPlan<T>::backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st, int set_comp) const {
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
+  const int64_t set_rows = (int64_t)(set_comp > 0 ? set_comp : ncomp) * Ny * Nz;
+  if (((int64_t)ncomp * Ny * Nz) % set_rows != 0) throw std::runtime_error("FFT: components not a multiple of the field size");
   strided_passes(S, ncomp, +1, st);
   const int shp = shape(0);
-  if (pencil_x_c2r<T, TO>(Nx, S, real, (int64_t)ncomp * Ny * Nz, reinterpret_cast<const cpx<T>*>(tw[0]), (T)scale,
-                          shp, st)) {
+  if (pencil_x_c2r<T, TO>(Nx, S, real, (int64_t)ncomp * Ny * Nz, set_rows, reinterpret_cast<const cpx<T>*>(tw[0]),
+                          (T)scale, shp, st)) {
     SCA_HIP(hipGetLastError());
     return;
   }
@@ -1144,7 +1151,7 @@ void FftPlan<T>::tune(int ncomp, hipStream_t st) {
       float best_ms = 1e30f;
       for (int c : cands) {
         auto run = [&] {
-          if (axis == 0) x_pass_fwd(real, true, S, ncomp, c, st);
+          if (axis == 0) x_pass_fwd(real, true, S, ncomp, c, st, 0);
           else strided_pass(S, ncomp, -1, axis, c, st);
         };
         run();  // warm-up
@@ -1183,12 +1190,12 @@ void FftPlan<T>::tune(int ncomp, hipStream_t st) {
 
 template struct FftPlan<double>;
 template struct FftPlan<float>;
-template void FftPlan<double>::forward<double>(const double*, double*, int, hipStream_t) const;
-template void FftPlan<double>::backward<double>(double*, double*, int, double, hipStream_t) const;
-template void FftPlan<float>::forward<double>(const double*, float*, int, hipStream_t) const;
-template void FftPlan<float>::backward<double>(float*, double*, int, double, hipStream_t) const;
-template void FftPlan<float>::forward<float>(const float*, float*, int, hipStream_t) const;
-template void FftPlan<float>::backward<float>(float*, float*, int, double, hipStream_t) const;
+template void FftPlan<double>::forward<double>(const double*, double*, int, hipStream_t, int) const;
+template void FftPlan<double>::backward<double>(double*, double*, int, double, hipStream_t, int) const;
+template void FftPlan<float>::forward<double>(const double*, float*, int, hipStream_t, int) const;
+template void FftPlan<float>::backward<double>(float*, double*, int, double, hipStream_t, int) const;
+template void FftPlan<float>::forward<float>(const float*, float*, int, hipStream_t, int) const;
+template void FftPlan<float>::backward<float>(float*, float*, int, double, hipStream_t, int) const;
 
 static GammaArgs gamma_args(const GammaHost& gh, const double* d_xi_tab, const double* d_aim) {
   GammaArgs g;

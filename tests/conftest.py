@@ -13,6 +13,29 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 PRODUCT_SO = os.path.join(ROOT, "solver_cpp_amd", "_build", "libsolver_cpp_amd.so")
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# Per-point fields after many increments: a rounding-level difference can move
+# a stopping decision of the Lp Newton or the state fixed point (both stop at
+# 1e-6 relative) in a few cells, and their neighbours follow through
+# equilibrium.  Two CPU builds of the oracle (with and without fused
+# multiply-add) differ from each other in this way: after the 40 increments
+# of the 64^3 / 200-grain case, P of 314 of 262144 cells (0.12 %) differs by
+# more than 1e-10 of max |P| between them, by up to 1.04e-9.  The GPU's share
+# of such cells has been 0.06 % and 0.24 % (two builds whose first increments
+# differed by rounding).  Required: at least 99.5 % of a seeded random cell
+# sample within 1e-10 of the field's maximum, and every sampled cell within
+# TAIL (ten times the largest difference seen, 1.36e-9).
+TAIL = 1e-8
+TAIL_FRACTION = 0.995
+
+
+def tail_close(got, want, n_random, what, rel=1e-10, tail=TAIL):
+    got, want = np.asarray(got), np.asarray(want)
+    err = np.abs(got - want).reshape(len(want), -1).max(axis=1) / np.abs(want).max()
+    frac = float((err[:n_random] <= rel).mean())
+    print(f"{what} within {rel:.0e} on {frac:.4f} of the random sample, max {err.max():.2e}", flush=True)
+    assert frac >= TAIL_FRACTION, (what, frac)
+    assert err.max() <= tail, (what, err.max())
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")

@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, PRODUCT_SO
+from conftest import GOLDEN, PRODUCT_SO, tail_close
 from solver_cpp_amd.capi import Library
 from solver_cpp_amd.geometry import EXAMPLE_NUMERICS
 from solver_cpp_amd.rve import tension_x
@@ -60,6 +60,25 @@ def test_batched_equals_alone(lib, precision):
             np.testing.assert_array_equal(x[k], y[k], err_msg=k)
 
 
+def test_batched_odd_grid_equals_alone(lib):
+    """Grids whose x pass pairs rows across an odd row count (27^3: 9 x 27 x
+    27 rows per RVE) joined in one transform: each RVE's rows are paired
+    among themselves, so the batched results are bitwise those alone."""
+    specs = batched_specs(lib, 4, seed=13, edges=(27,), grains=(4, 8), **EXAMPLE_NUMERICS)
+    loads = [[tension_x(t=1.0, N=2, rot_q=q)] for _, q in specs]
+    alone, fa, _ = _run(lib, specs, loads, 2, batched=False, workers=1)
+    batch, fb, st = _run(lib, specs, loads, 2, batched=True, workers=4)
+    assert st.max_batch >= 2, st.max_batch
+    for a_job, b_job in zip(alone, batch):
+        for a, b in zip(a_job, b_job):
+            assert a.status == 0 and b.status == 0
+            assert (a.iterations, a.evaluations) == (b.iterations, b.evaluations)
+            np.testing.assert_array_equal(np.array(a.P_av), np.array(b.P_av))
+    for x, y in zip(fa, fb):
+        for k in x:
+            np.testing.assert_array_equal(x[k], y[k], err_msg=k)
+
+
 C4_FIXTURE = os.path.join(GOLDEN, "c4_fixture.npz")
 
 
@@ -70,8 +89,9 @@ def test_c4_batched_against_oracle(lib):
     run together through the batched server under a memory budget that
     admits only some of them at once, against the oracle's runs of the same
     RVEs: identical iterations and evaluations in every increment, <P>
-    within 1e-10 relative at every increment, P and F within 1e-10 on the
-    sampled cells after the last increment."""
+    within 1e-10 relative at every increment, P and F on the sampled cells
+    after the last increment by tail_close (stopping decisions of the
+    material point moved by rounding in a few cells)."""
     fx = np.load(C4_FIXTURE)
     specs = batched_specs(lib, 64, seed=7, **EXAMPLE_NUMERICS)
     chosen = [specs[i] for i in fx["index"]]
@@ -111,9 +131,7 @@ def test_c4_batched_against_oracle(lib):
         cells = fx[f"cells_{j}"]
         for k in ("P", "F"):
             got = fields[j][k].reshape(spec.N, 9)[cells]
-            want = fx[f"{k}_{j}"]
-            err = np.abs(got - want).max() / np.abs(want).max()
-            assert err < 1e-10, f"rve {j} {k}: {err:.2e}"
+            tail_close(got, fx[f"{k}_{j}"], len(cells), f"rve {j} {k} after 10 increments")
 
 
 C5_FIXTURE = os.path.join(GOLDEN, "c5_fixture.npz")
