@@ -280,6 +280,24 @@ def test_first_increment_homogeneous_forward(product, oracle):
     close(a.field("P"), b.field("P"), REL, "P")
 
 
+@pytest.mark.parametrize("cells", [(27, 18, 24), (20, 14, 22)])
+def test_fft_plan_modes_give_the_same_bits(product, cells):
+    """fft_plan_mode 1 (launch shapes timed per grid, like FFTW_MEASURE) and 0
+    (built-in shapes, like FFTW_ESTIMATE) on pencil lengths (27, 18, 24) and
+    generic ones (20, 14, 22): the same counts and bitwise the same fields."""
+    mat, ori, size = polycrystal(cells, 6, seed=3, ori_seed=4)
+    out = []
+    for mode in (0, 1):
+        r = Rve(product, cells, size, mat, ori, fft_plan_mode=mode, **EXAMPLE_NUMERICS)
+        r.init()
+        incs = [(d["iterations"], d["evaluations"]) for d, _ in zip(r.increments([tension_x(t=1.0, N=2)]), range(2))]
+        out.append((incs, r.field("F"), r.field("P")))
+        r.close()
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
 @pytest.mark.parametrize("cells", [(8, 6, 4), (8, 6, 16), (7, 5, 12), (9, 8, 32), (6, 4, 64)])
 def test_init_and_single_residual(product, oracle, cells):
     """Init, a solution and one residual at a perturbed F, over grids with odd
