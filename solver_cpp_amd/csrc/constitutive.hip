@@ -816,27 +816,37 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
             store_integrated(sh.A, sh.iFp0, t, sdt, Lp, S, io);
             break;
           }
+          // the rate history is read from LDS in one batch before the sums
+          // (issued together: one wait instead of one per state variable)
+          double l1v[NSTATE], d12v[NSTATE];
+#pragma unroll
+          for (int k = 0; k < NSTATE; ++k) {
+            l1v[k] = h_last1[k][t];
+            d12v[k] = h_d12[k][t];
+          }
           double dp12 = 0.0, dp22 = 0.0, dp01 = 0.0;
 #pragma unroll
           for (int k = 0; k < NSTATE; ++k) {
-            const double l1 = h_last1[k][t], d12 = h_d12[k][t];
-            dp12 += (dot[k] - l1) * d12;
-            dp22 += d12 * d12;
-            dp01 += dot[k] * l1;
+            dp12 += (dot[k] - l1v[k]) * d12v[k];
+            dp22 += d12v[k] * d12v[k];
+            dp01 += dot[k] * l1v[k];
           }
           const double zeta =
               (fmin(dp01, dp12) < 0.0 && dp22 > 0.0) ? 0.75 + 0.25 * tanh(2.0 + 4.0 * dp12 / dp22) : 1.0;
+          double gv[NSLIP];  // the shear column, read in one batch as well
+#pragma unroll
+          for (int k = 0; k < NSLIP; ++k) gv[k] = h_gam[k][t];
           bool conv = true;
 #pragma unroll
           for (int k = 0; k < NSTATE; ++k) {
-            const double l1 = h_last1[k][t];
+            const double l1 = l1v[k];
             // the damped rate replaces dotState: it is dotState_last(:,1) of
             // the next iteration (phase_mechanical.f90:653-655)
             const double dk = dot[k] * zeta + l1 * (1.0 - zeta);
             dot[k] = dk;
             h_d12[k][t] = dk - l1;  // the next iteration's shift
             h_last1[k][t] = dk;
-            const double cur = (k < NSLIP) ? xi[k] : h_gam[k - NSLIP][t];
+            const double cur = (k < NSLIP) ? xi[k] : gv[k - NSLIP];
             const double r = cur - s0[k] - dk * sdt;
             const double nw = cur - r;
             if (k < NSLIP) xi[k] = nw;
@@ -866,10 +876,13 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
   }
   store33(s.Lp, N, l, Lp);
   store33(s.S, N, l, S);
+  double gv[NSLIP];  // the shear column, read from LDS in one batch
+#pragma unroll
+  for (int k = 0; k < NSLIP; ++k) gv[k] = h_gam[k][t];
 #pragma unroll
   for (int k = 0; k < NSLIP; ++k) {
     st(s.xi + (k) * N, l, xi[k]);
-    st(s.gamma + (k) * N, l, h_gam[k][t]);
+    st(s.gamma + (k) * N, l, gv[k]);
   }
   if (s.iters) {
     s.iters[c] = newton;
