@@ -69,7 +69,7 @@ void Batcher::leave() {
   cv_req_.notify_all();  // the others may all be waiting now
 }
 
-void Batcher::evaluate(Rve* rve, const double* F, double* r) {
+void Batcher::evaluate(Rve* rve, const double* F, void* r) {
   Req q{rve, F, r};
   std::unique_lock<std::mutex> lk(m_);
   if (waiting_.empty()) first_ = std::chrono::steady_clock::now();
@@ -212,8 +212,10 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
         const Req& q = *reqs[idx[c0 + j]];
         Rve& v = *q.rve;
         double* slot = g.real + (size_t)9 * N * j;
-        gather[j] = BatchCopy{v.mp.P, slot, 9 * N};
-        scatter[j] = BatchCopy{slot, q.r, 9 * N};
+        gather[j] = BatchCopy{v.mp.P, slot, 2 * 9 * N};
+        // an FP32 group's residuals are floats (the FP32 stage stores them so)
+        scatter[j] = g.fp32 ? BatchCopy{static_cast<float*>(static_cast<void*>(g.real)) + (size_t)9 * N * j, q.r, 9 * N}
+                            : BatchCopy{slot, q.r, 2 * 9 * N};
         gam[j] = GammaBatchItem{static_cast<char*>(g.spec) + spec_stride * j, v.gamma_host(true), v.small + SM_AIM,
                                 v.d_xi_tab, v.partial};
         div[j] = BatchReduce{v.partial, gamma_blocks(K), 1, v.small + SM_DIV};
@@ -229,7 +231,8 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
         launch_gamma_div_batch<double>(gam.data(), m, K, ta_, st_);
       }
       launch_final_reduce_batch(div.data(), m, ta_, st_);
-      if (g.fp32) g.f32.backward(static_cast<float*>(g.spec), g.real, 9 * m, 1.0, st_, 9);
+      if (g.fp32) g.f32.backward(static_cast<float*>(g.spec), static_cast<float*>(static_cast<void*>(g.real)), 9 * m,
+                                 1.0, st_, 9);
       else g.f64.backward(static_cast<double*>(g.spec), g.real, 9 * m, 1.0, st_, 9);
       launch_copy_batch(scatter.data(), m, ta_, st_);
     }
@@ -248,7 +251,7 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
     scal_cap_ = cap;
   }
   std::vector<BatchCopy> sc(n);
-  for (int i = 0; i < n; ++i) sc[i] = BatchCopy{reqs[i]->rve->small + SM_SUMS, d_scal_ + (size_t)NS * i, NS};
+  for (int i = 0; i < n; ++i) sc[i] = BatchCopy{reqs[i]->rve->small + SM_SUMS, d_scal_ + (size_t)NS * i, 2 * NS};
   launch_copy_batch(sc.data(), n, ta_, st_);
   SCA_HIP(hipMemcpyAsync(h_scal_, d_scal_, (size_t)NS * n * sizeof(double), hipMemcpyDeviceToHost, st_));
   SCA_HIP(hipStreamSynchronize(st_));

@@ -31,16 +31,17 @@ class Batcher {
   void join();   // a context starts an increment: its evaluations are batched
   void leave();  // ... and ends it
   // r = form_residual(F) of rve, computed in a joint launch; F must be
-  // complete (the caller synchronised its stream).  Returns once r, the
+  // complete (the caller synchronised its stream).  r holds floats while
+  // rve is in its FP32 stage, doubles otherwise.  Returns once r, the
   // material-point state and rve->h_scal hold the results.
-  void evaluate(Rve* rve, const double* F, double* r);
+  void evaluate(Rve* rve, const double* F, void* r);
   BatchStats stats();
 
  private:
   struct Req {
     Rve* rve;
     const double* F;
-    double* r;
+    void* r;
     bool done = false;
     std::string err;
   };
@@ -49,7 +50,7 @@ class Batcher {
     int cells[3] = {0, 0, 0};
     FftPlan<double> f64;
     FftPlan<float> f32;
-    double* real = nullptr;  // 9 N doubles per RVE slot
+    double* real = nullptr;  // 9 N doubles per RVE slot (P in; r out, as floats in an FP32 group)
     void* spec = nullptr;    // 9 K complex per RVE slot
     int cap = 0;
     int tuned_for = 0;  // components / 9 the launch shapes were measured for

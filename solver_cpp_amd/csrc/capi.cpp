@@ -564,7 +564,7 @@ int sc_ngmres_replay(int64_t n, int m, const double* x0, int n_evals, const doub
     DBuf X(n);
     X.put(x0, s.s);
     int e = 0;
-    const int reason = ng.solve(
+    const int reason = ng.solve<double>(
         X.p, m,
         [&](const double* x, double* f) {
           if (e >= n_evals) throw std::runtime_error("replay ran past the recorded evaluations");

@@ -73,11 +73,16 @@ class Ngmres {
  public:
   Ngmres(int64_t n, hipStream_t st, KTimer* kt = nullptr);
   ~Ngmres();
-  // residual(x, f): f = F(x); conv(k): 1 converged, -1 diverged, 0 iterating
-  int solve(double* X, int msize, const std::function<void(const double*, double*)>& residual,
+  // residual(x, f): f = F(x); conv(k): 1 converged, -1 diverged, 0 iterating.
+  // TF: element type of the residuals (float for a residual computed in
+  // FP32, which it holds exactly); iterates, inner products and the
+  // combination are FP64.
+  template <typename TF>
+  int solve(double* X, int msize, const std::function<void(const double*, TF*)>& residual,
             const std::function<int(int)>& conv, const std::function<void(int)>& set_iter);
-  std::vector<double> reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
-                             const std::vector<const double*>& b);
+  template <typename T>
+  std::vector<double> reduce(const std::vector<int>& ops, const std::vector<const T*>& a,
+                             const std::vector<const T*>& b);
   // final sums of the first nout partial rows, to the host
   std::vector<double> collect(int nout, int64_t units);
   // called after every synchronisation of the reductions (the owner takes up
@@ -93,7 +98,8 @@ class Ngmres {
  private:
   // The vectors of a solve are buffers of one pool, referenced by the
   // iterate (Xc, its residual Fv), the trial points and residuals of an
-  // iteration and the history slots.  Accepting a point or recording it in
+  // iteration and the history slots (a residual buffer holds n values of
+  // the solve's residual type).  Accepting a point or recording it in
   // the history moves a reference instead of copying the vector; a buffer
   // is written only while nothing else references it.  The caller's X is
   // referenced until the first accepted point and receives the solution at
@@ -125,9 +131,10 @@ class Rve {
   sc_solution calculate_solution();
   int run_loadcase(const sc_loadstep* steps, int n_steps, int* inc_iterations, int* inc_evaluations, int max_incs);
   int next_increment(const sc_loadstep* steps, int n_steps, LoadCursor& cur, int* iterations, int* evaluations);
-  void form_residual(const double* Fdev, double* rdev);
-  void launch_residual(const double* Fdev, double* rdev);
-  void launch_spectral(double* rdev);
+  // r: floats during the FP32 stage of the mixed form (use32), doubles otherwise
+  void form_residual(const double* Fdev, void* rdev);
+  void launch_residual(const double* Fdev, void* rdev);
+  void launch_spectral(void* rdev);
   bool ensure_residual_graph();  // false: evaluate eagerly this time
   void release_residual_graph();
   void constitutive_response(const double* Fdev, double dt);

@@ -581,13 +581,15 @@ void Rve::tangent() {
   kt.begin(SC_KT_TANGENT, st);
   launch_mp_tangent(mp, mc, last_dt, dPdF, st);
   kt.end(SC_KT_TANGENT, N, st);
-  std::vector<int> ops(81, 2);
-  std::vector<const double*> a(81);
-  for (int q = 0; q < 81; ++q) a[q] = dPdF + q * N;
-  std::vector<double> sums = reduce(ops, a, {}, N);
-  for (int q = 0; q < 81; ++q) C_volAvg[q] = sums[q] * wgt;
+  // one pass over dPdF: the cells of the extreme norms and the plane sums
   int64_t imx = -1, imn = -1;
-  tangent_minmax_select(dPdF, N, d_n2, partial, d_minmax_i, imx, imn, st);
+  const int rb = red_blocks();
+  tangent_minmax_select(dPdF, N, d_n2, partial + (size_t)81 * rb, d_minmax_i, imx, imn, st, partial);
+  double sums[81];
+  launch_final_reduce(partial, 81, rb, small, st);
+  SCA_HIP(hipMemcpyAsync(sums, small, sizeof sums, hipMemcpyDeviceToHost, st));
+  SCA_HIP(hipStreamSynchronize(st));
+  for (int q = 0; q < 81; ++q) C_volAvg[q] = sums[q] * wgt;
   double dmax[81], dmin[81];
   for (int q = 0; q < 81; ++q) {
     dmax[q] = 0.0;
@@ -630,7 +632,7 @@ GammaHost Rve::gamma_host(bool do_gamma) const {
 // inverse transform into r, and the asynchronous copy of the response
 // scalars.  Its launches depend on F, r, delta_t, the compliance S, the load
 // rotation, C_ref and the precision phase only, so it can be captured.
-void Rve::launch_residual(const double* Fdev, double* rdev) {
+void Rve::launch_residual(const double* Fdev, void* rdev) {
   MpState s = mp;
   s.F = const_cast<double*>(Fdev);  // read only
   kt.begin(SC_KT_RESPONSE, st);
@@ -653,14 +655,14 @@ void Rve::launch_residual(const double* Fdev, double* rdev) {
 // update: the aim of the convolution, forward transform, Gamma convolution
 // with the divergence, inverse transform into r, and the asynchronous copy of
 // the response scalars.
-void Rve::launch_spectral(double* rdev) {
+void Rve::launch_spectral(void* rdev) {
   // DC of the convolution: rotated S : P_av, computed on the device
   double R[9], Rrm[9];
   qu2om_host(rot_q, R);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Rrm[3 * i + j] = R[i + 3 * j];
   launch_bc_aim(small + SM_SUMS, S, Rrm, wgt, 1, small + SM_AIM, st);
-  if (use32) {  // FP32 transforms and spectrum; P in and r out stay FP64 planes
+  if (use32) {  // FP32 transforms and spectrum from the FP64 P; r out in FP32 (exactly the transform's values)
     kt.begin(SC_KT_FFT_FWD, st);
     fft32.forward(mp.P, spec32, 9, st);
     kt.end(SC_KT_FFT_FWD, N, st);
@@ -669,7 +671,7 @@ void Rve::launch_spectral(double* rdev) {
     launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
     kt.end(SC_KT_GAMMA, K, st);
     kt.begin(SC_KT_FFT_BWD, st);
-    fft32.backward(spec32, rdev, 9, 1.0, st);
+    fft32.backward(spec32, static_cast<float*>(rdev), 9, 1.0, st);
     kt.end(SC_KT_FFT_BWD, N, st);
   } else {
     kt.begin(SC_KT_FFT_FWD, st);
@@ -680,7 +682,7 @@ void Rve::launch_spectral(double* rdev) {
     launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
     kt.end(SC_KT_GAMMA, K, st);
     kt.begin(SC_KT_FFT_BWD, st);
-    fft.backward(spec, rdev, 9, 1.0, st);
+    fft.backward(spec, static_cast<double*>(rdev), 9, 1.0, st);
     kt.end(SC_KT_FFT_BWD, N, st);
   }
   static_assert(SM_ILL == SM_SUMS + 9 && SM_DIV == SM_ILL + 1 && SM_ITERS == SM_DIV + 1,
@@ -764,7 +766,7 @@ void Rve::release_residual_graph() {
 // FP32 stage returned was its last evaluation), so P and the material-point
 // state are those of F already; the evaluation runs the FP64 spectral stages
 // on them and counts no material-point update.
-void Rve::form_residual(const double* Fdev, double* rdev) {
+void Rve::form_residual(const double* Fdev, void* rdev) {
   if (nfuncs == 0 && snes_iter == 0) total_iter = iter_base - 1;
   if (total_iter - iter_base <= snes_iter) total_iter += 1;
   const bool reused = reuse_response;
@@ -777,7 +779,8 @@ void Rve::form_residual(const double* Fdev, double* rdev) {
   } else if (g_in && !kt.on && ensure_residual_graph()) {
     SCA_HIP(hipMemcpyAsync(g_in, Fdev, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
     SCA_HIP(hipGraphLaunch(g_exec, st));
-    SCA_HIP(hipMemcpyAsync(rdev, g_out, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    SCA_HIP(hipMemcpyAsync(rdev, g_out, 9 * N * (use32 ? sizeof(float) : sizeof(double)), hipMemcpyDeviceToDevice,
+                           st));
   } else {
     launch_residual(Fdev, rdev);
   }
@@ -850,15 +853,16 @@ double* Ngmres::take() {
   pool.push_back(p);
   return p;
 }
-std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vector<const double*>& a,
-                                   const std::vector<const double*>& b) {
+template <typename T>
+std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vector<const T*>& a,
+                                   const std::vector<const T*>& b) {
   const int nvec = (int)ops.size();
   std::vector<double> out(nvec);
   for (int v0 = 0; v0 < nvec; v0 += MAX_RED_VEC) {
     const int nv = std::min(MAX_RED_VEC, nvec - v0);
     if (kt) kt->begin(SC_KT_REDUCE, st);
-    launch_partial_reduce<double>(ops.data() + v0, n, nv, a.data() + v0, b.empty() ? nullptr : b.data() + v0,
-                                  partial, st);
+    launch_partial_reduce<T>(ops.data() + v0, n, nv, a.data() + v0, b.empty() ? nullptr : b.data() + v0, partial,
+                             st);
     launch_final_reduce(partial, nv, red_blocks(), small, st);
     if (kt) kt->end(SC_KT_REDUCE, n * nv, st);
     SCA_HIP(hipMemcpyAsync(out.data() + v0, small, nv * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -880,11 +884,14 @@ std::vector<double> Ngmres::collect(int nout, int64_t units) {
   return out;
 }
 
-int Ngmres::solve(double* X, int msize, const std::function<void(const double*, double*)>& residual,
+template <typename TF>
+int Ngmres::solve(double* X, int msize, const std::function<void(const double*, TF*)>& residual,
                   const std::function<int(int)>& conv, const std::function<void(int)>& set_iter) {
   const int max_its = 50;
   const double gammaA = 2.0, gammaC = 2.0, epsilonB = 0.1, deltaB = 0.9;
   const int restart_it = 2;
+  // residual buffers hold TF values
+  auto tf = [](const double* p) { return reinterpret_cast<const TF*>(p); };
   auto Fdot = [&](int i) { return Fh[i]; };
   auto Xdot = [&](int i) { return Xh[i]; };
   std::vector<double> fnorms(msize, 0.0), Q(msize * msize, 0.0);
@@ -897,7 +904,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
   const double* last_x = nullptr;
   auto eval = [&](const double* x, double* f) {
     last_x = x;
-    residual(x, f);
+    residual(x, reinterpret_cast<TF*>(f));
   };
   auto done = [&](int r) {
     solution_evaluated_last = Xc == last_x;
@@ -910,7 +917,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
   Fv = take();
   set_iter(0);
   eval(Xc, Fv);
-  double fnorm = std::sqrt(reduce({0}, {Fv}, {Fv})[0]);
+  double fnorm = std::sqrt(reduce<TF>({0}, {tf(Fv)}, {tf(Fv)})[0]);
   if (std::isnan(fnorm) || std::isinf(fnorm)) return done(-4);  // SNES_DIVERGED_FNORM_NAN
   double fminnorm = fnorm;
   int reason = conv(0);
@@ -927,7 +934,7 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
     XM = nullptr;
     XM = take();
     if (kt) kt->begin(SC_KT_REDUCE, st);
-    launch_axpby(1.0, Xc, -1.0, Fv, XM, n, st);  // basic line search: XM = X - F
+    launch_axpby(1.0, Xc, -1.0, tf(Fv), XM, n, st);  // basic line search: XM = X - F
     if (kt) kt->end(SC_KT_REDUCE, n, st);
     FM = nullptr;
     FM = take();
@@ -936,11 +943,11 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
     // Fdot_ivec and the history (each vector read once)
     std::vector<double> dots(1 + 2 * l);
     if (2 * (l + 1) <= MAX_RED_VEC) {
-      std::vector<const double*> vs(l + 1);
-      for (int i = 0; i < l; ++i) vs[i] = Fdot(i);
-      vs[l] = FM;
+      std::vector<const TF*> vs(l + 1);
+      for (int i = 0; i < l; ++i) vs[i] = tf(Fdot(i));
+      vs[l] = tf(FM);
       if (kt) kt->begin(SC_KT_REDUCE, st);
-      launch_shared_reduce(0, n, FM, Fdot(ivec), l + 1, vs.data(), partial, st);
+      launch_shared_reduce<TF>(0, n, tf(FM), tf(Fdot(ivec)), l + 1, vs.data(), partial, st);
       const std::vector<double> r = collect(2 * (l + 1), n * (l + 3));
       dots[0] = r[2 * l];
       for (int i = 0; i < l; ++i) {
@@ -949,16 +956,16 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
       }
     } else {
       std::vector<int> ops(1 + 2 * l, 0);
-      std::vector<const double*> va(1 + 2 * l), vb(1 + 2 * l);
-      va[0] = FM;
-      vb[0] = FM;
+      std::vector<const TF*> va(1 + 2 * l), vb(1 + 2 * l);
+      va[0] = tf(FM);
+      vb[0] = tf(FM);
       for (int i = 0; i < l; ++i) {
-        va[1 + i] = FM;
-        vb[1 + i] = Fdot(i);
-        va[1 + l + i] = Fdot(ivec);
-        vb[1 + l + i] = Fdot(i);
+        va[1 + i] = tf(FM);
+        vb[1 + i] = tf(Fdot(i));
+        va[1 + l + i] = tf(Fdot(ivec));
+        vb[1 + l + i] = tf(Fdot(i));
       }
-      dots = reduce(ops, va, vb);
+      dots = reduce<TF>(ops, va, vb);
     }
     const double fMnorm = std::sqrt(dots[0]);
     bool fused = false;
@@ -1004,32 +1011,33 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
     if (fused) {
       if (kt) kt->begin(SC_KT_REDUCE, st);
       const int op0 = 0;
-      const double* fa = FA;
-      launch_partial_reduce<double>(&op0, n, 1, &fa, &fa, partial, st);
+      const TF* fa = tf(FA);
+      launch_partial_reduce<TF>(&op0, n, 1, &fa, &fa, partial, st);
       nrm = collect(2 + l, n);
     } else if (2 + l <= MAX_RED_VEC) {
       if (kt) kt->begin(SC_KT_REDUCE, st);
       const int op0 = 0;
-      const double* fa = FA;
-      launch_partial_reduce<double>(&op0, n, 1, &fa, &fa, partial, st);
+      const TF* fa = tf(FA);
+      launch_partial_reduce<TF>(&op0, n, 1, &fa, &fa, partial, st);
       std::vector<const double*> vs(l + 1);
       vs[0] = XM;
       for (int i = 0; i < l; ++i) vs[1 + i] = Xdot(i);
-      launch_shared_reduce(1, n, XA, nullptr, l + 1, vs.data(), partial + red_blocks(), st);
+      launch_shared_reduce<double>(1, n, XA, nullptr, l + 1, vs.data(), partial + red_blocks(), st);
       nrm = collect(2 + l, n * (l + 3));
     } else {
-      std::vector<int> ops2(2 + l, 1);
-      std::vector<const double*> wa(2 + l), wb(2 + l);
-      ops2[0] = 0;
-      wa[0] = FA;
-      wb[0] = FA;
-      wa[1] = XA;
-      wb[1] = XM;
+      // ||FA||^2 apart (another element type), then the distances; each
+      // reduction is that vector's own, whatever it is launched with
+      nrm = reduce<TF>({0}, {tf(FA)}, {tf(FA)});
+      std::vector<int> ops2(1 + l, 1);
+      std::vector<const double*> wa(1 + l), wb(1 + l);
+      wa[0] = XA;
+      wb[0] = XM;
       for (int i = 0; i < l; ++i) {
-        wa[2 + i] = Xdot(i);
-        wb[2 + i] = XA;
+        wa[1 + i] = Xdot(i);
+        wb[1 + i] = XA;
       }
-      nrm = reduce(ops2, wa, wb);
+      const std::vector<double> d = reduce<double>(ops2, wa, wb);
+      nrm.insert(nrm.end(), d.begin(), d.end());
     }
     const double fAnorm = std::sqrt(nrm[0]);
     const double dnorm = std::sqrt(nrm[1]);
@@ -1079,11 +1087,20 @@ int Ngmres::solve(double* X, int msize, const std::function<void(const double*, 
   return done(-5);  // SNES_DIVERGED_MAX_IT
 }
 
+template int Ngmres::solve<double>(double*, int, const std::function<void(const double*, double*)>&,
+                                   const std::function<int(int)>&, const std::function<void(int)>&);
+template int Ngmres::solve<float>(double*, int, const std::function<void(const double*, float*)>&,
+                                  const std::function<int(int)>&, const std::function<void(int)>&);
+
 int Rve::snes_ngmres() {
   const int msize = desc.grid_num.ngmres_m > 0 ? desc.grid_num.ngmres_m : 30;
   snes_iter = 0;
   nfuncs = 0;
-  return ngm->solve(
+  if (use32)
+    return ngm->solve<float>(
+        X, msize, [this](const double* x, float* f) { form_residual(x, f); }, [this](int) { return converged(); },
+        [this](int k) { snes_iter = k; });
+  return ngm->solve<double>(
       X, msize, [this](const double* x, double* f) { form_residual(x, f); }, [this](int) { return converged(); },
       [this](int k) { snes_iter = k; });
 }

@@ -75,14 +75,18 @@ void launch_partial_reduce(const int* ops, int64_t n, int nvec, const T* const* 
 // mode 0: out[2i] = <u, v_i>, out[2i+1] = <w, v_i>; mode 1: out[i] = sum (v_i - u)^2,
 // for i < nv, as partials (no*nv x red_blocks()) for launch_final_reduce; each
 // result bitwise equal to launch_partial_reduce's for the same pair
-void launch_shared_reduce(int mode, int64_t n, const double* u, const double* w, int nv, const double* const* v,
+template <typename T>
+void launch_shared_reduce(int mode, int64_t n, const T* u, const T* w, int nv, const T* const* v,
                           double* partial, hipStream_t st);
 // cells of C_minMaxAvg: first cell of largest / smallest |dPdF|^2 with norms
 // within TANGENT_TIE_REL of the extreme counted as tied.  n2: N doubles of
 // scratch, pv: 2*red_blocks() doubles, pi: 2*red_blocks() long longs.
+// sums (optional, apart from pv): the same pass also leaves the 81 planes'
+// partial sums there (81 x red_blocks(), as launch_partial_reduce op 2).
 constexpr double TANGENT_TIE_REL = 1e-10;
 void tangent_minmax_select(const double* dPdF, int64_t N, double* n2, double* pv, long long* pi, int64_t& imax,
-                           int64_t& imin, hipStream_t st);
+                           int64_t& imin, hipStream_t st,
+                           double* sums = nullptr);
 void launch_gather81(const double* d, int64_t N, int64_t c, double* out, hipStream_t st);
 void launch_bc_aim(const double* sums, const double S[81], const double R[9], double wgt, int rotate, double* aim,
                    hipStream_t st);
@@ -94,6 +98,7 @@ void launch_fwd_diff(const double* sums, double wgt, const double* aim, double* 
 void launch_rate(int het, const double* f0, const double* f, double dt, const double* av, double* out, int64_t N,
                  hipStream_t st);
 void launch_axpby(double a, const double* x, double b, const double* y0, double* y, int64_t n, hipStream_t st);
+void launch_axpby(double a, const double* x, double b, const float* y0, double* y, int64_t n, hipStream_t st);
 void launch_lincomb(const double* XM, double c0, int l, const double* beta, const double* const* X, double* XA,
                     int64_t n, hipStream_t st);
 // XA as launch_lincomb plus the partials (l+1 rows, red_blocks() each) of
@@ -115,11 +120,11 @@ struct BatchReduce {
   double* out;
 };
 void launch_final_reduce_batch(const BatchReduce* items, int n, TableArena& ta, hipStream_t st);
-// dst[i] = src[i] for i < n, per entry
+// bitwise copies of `words` 32-bit words per entry (a double is two)
 struct BatchCopy {
-  const double* src;
-  double* dst;
-  int64_t n;
+  const void* src;
+  void* dst;
+  int64_t words;
 };
 void launch_copy_batch(const BatchCopy* items, int n, TableArena& ta, hipStream_t st);
 // the DC aim S : (R P_av R^T) of each RVE (launch_bc_aim)

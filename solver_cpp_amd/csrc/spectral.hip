@@ -541,21 +541,24 @@ __global__ void __launch_bounds__(256) partial_reduce_kernel(RedArgs r, double* 
   double s = 0.0;
   constexpr int64_t STRIDE = (int64_t)RED_BLOCKS * 256;
   int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  // four grid-stride elements per trip with their loads issued first; the
-  // sum is accumulated in the same element order as one per trip
-  for (; t + 3 * STRIDE < r.n; t += 4 * STRIDE) {
-    double x[4], y[4];
+  // U grid-stride elements per trip with their loads issued first (32 bytes
+  // per thread and vector in flight); the sum is accumulated in the same
+  // element order as one per trip
+  constexpr int U = 32 / sizeof(T) < 4 ? 4 : 32 / sizeof(T);
+  for (; t + (U - 1) * STRIDE < r.n; t += U * STRIDE) {
+    T x[U], y[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) x[u] = (double)a[t + u * STRIDE];
+    for (int u = 0; u < U; ++u) x[u] = a[t + u * STRIDE];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) y[u] = own ? x[u] : (double)b[t + u * STRIDE];
+    for (int u = 0; u < U; ++u) y[u] = own ? x[u] : b[t + u * STRIDE];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (op == 0) s += x[u] * y[u];
+    for (int u = 0; u < U; ++u) {
+      const double xu = (double)x[u], yu = (double)y[u];
+      if (op == 0) s += xu * yu;
       else if (op == 1) {
-        const double d = x[u] - y[u];
+        const double d = xu - yu;
         s += d * d;
-      } else s += x[u];
+      } else s += xu;
     }
   }
   for (; t < r.n; t += STRIDE) {
@@ -583,76 +586,87 @@ __global__ void __launch_bounds__(256) partial_reduce_kernel(RedArgs r, double* 
 // order and the block tree are those of partial_reduce_kernel, so every
 // result is bitwise the one of the corresponding single reduction.
 constexpr int SHARED_GROUP = 16;
+template <typename T>
 struct SharedRedArgs {
   int64_t n;
-  const double* u;
-  const double* w;
+  const T* u;
+  const T* w;
   int nv;  // vectors in this launch, 1..SHARED_GROUP
-  const double* v[SHARED_GROUP];
+  const T* v[SHARED_GROUP];
 };
 
-template <int MODE>
-__global__ void __launch_bounds__(256) shared_reduce_kernel(SharedRedArgs r, double* partial) {
+// T: element type of the vectors (float vectors are summed in double, as the
+// same values held in double would be).  NV: vectors in the launch (loads
+// without branches, registers for exactly these); a vector equal to u (the
+// N-GMRES trial residual dotted with itself) is taken from u's own load.
+template <int MODE, typename T, int NV>
+__global__ void __launch_bounds__(256) shared_reduce_kernel(SharedRedArgs<T> r, double* partial) {
   constexpr int NO = MODE == 0 ? 2 : 1;  // outputs per vector
-  constexpr int NACC = NO * SHARED_GROUP;
+  constexpr int NACC = NO * NV;
   constexpr int CH = 8;  // accumulators per block-tree pass
+  // elements per trip: more when the vectors are few (bytes in flight)
+  constexpr int U = NV <= 4 ? 8 : (NV <= 8 ? 4 : 2);
   __shared__ double red[CH][256];
   const int t = threadIdx.x;
   double s[NACC];
 #pragma unroll
   for (int j = 0; j < NACC; ++j) s[j] = 0.0;
+  bool self[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) self[i] = r.v[i] == r.u;
   constexpr int64_t STRIDE = (int64_t)RED_BLOCKS * 256;
   int64_t e = (int64_t)blockIdx.x * 256 + t;
-  // two grid-stride elements per trip, loads first, same accumulation order
-  for (; e + STRIDE < r.n; e += 2 * STRIDE) {
-    double x[2], y[2], v[SHARED_GROUP][2];
+  // U grid-stride elements per trip, loads first, the sums accumulated in
+  // the same element order as one per trip
+  for (; e + (U - 1) * STRIDE < r.n; e += U * STRIDE) {
+    T x[U], y[U], v[NV][U];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < U; ++u) {
       x[u] = r.u[e + u * STRIDE];
-      y[u] = MODE == 0 ? r.w[e + u * STRIDE] : 0.0;
+      y[u] = MODE == 0 ? r.w[e + u * STRIDE] : (T)0;
     }
 #pragma unroll
-    for (int i = 0; i < SHARED_GROUP; ++i)
+    for (int i = 0; i < NV; ++i)
+      if (!self[i]) {
 #pragma unroll
-      for (int u = 0; u < 2; ++u) v[i][u] = (i < r.nv) ? r.v[i][e + u * STRIDE] : 0.0;
+        for (int u = 0; u < U; ++u) v[i][u] = r.v[i][e + u * STRIDE];
+      } else {
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < U; ++u) v[i][u] = x[u];
+      }
 #pragma unroll
-      for (int i = 0; i < SHARED_GROUP; ++i) {
-        if (i < r.nv) {
-          if constexpr (MODE == 0) {
-            s[2 * i] += x[u] * v[i][u];
-            s[2 * i + 1] += y[u] * v[i][u];
-          } else {
-            const double d = v[i][u] - x[u];
-            s[i] += d * d;
-          }
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int i = 0; i < NV; ++i) {
+        const double xu = (double)x[u], vi = (double)v[i][u];
+        if constexpr (MODE == 0) {
+          s[2 * i] += xu * vi;
+          s[2 * i + 1] += (double)y[u] * vi;
+        } else {
+          const double d = vi - xu;
+          s[i] += d * d;
         }
       }
   }
   for (; e < r.n; e += STRIDE) {
-    const double x = r.u[e];
-    const double y = MODE == 0 ? r.w[e] : 0.0;
+    const double x = (double)r.u[e];
+    const double y = MODE == 0 ? (double)r.w[e] : 0.0;
 #pragma unroll
-    for (int i = 0; i < SHARED_GROUP; ++i) {
-      if (i < r.nv) {
-        const double v = r.v[i][e];
-        if constexpr (MODE == 0) {
-          s[2 * i] += x * v;
-          s[2 * i + 1] += y * v;
-        } else {
-          const double d = v - x;
-          s[i] += d * d;
-        }
+    for (int i = 0; i < NV; ++i) {
+      const double v = self[i] ? x : (double)r.v[i][e];
+      if constexpr (MODE == 0) {
+        s[2 * i] += x * v;
+        s[2 * i + 1] += y * v;
+      } else {
+        const double d = v - x;
+        s[i] += d * d;
       }
     }
   }
-  const int nout = NO * r.nv;
 #pragma unroll
   for (int c0 = 0; c0 < NACC; c0 += CH) {
-    if (c0 >= nout) break;
 #pragma unroll
-    for (int k = 0; k < CH; ++k) red[k][t] = s[c0 + k];
+    for (int k = 0; k < CH; ++k) red[k][t] = c0 + k < NACC ? s[c0 + k] : 0.0;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
       if (t < w) {
@@ -661,7 +675,7 @@ __global__ void __launch_bounds__(256) shared_reduce_kernel(SharedRedArgs r, dou
       }
       __syncthreads();
     }
-    if (t < CH && c0 + t < nout) partial[(int64_t)(c0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
+    if (t < CH && c0 + t < NACC) partial[(int64_t)(c0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
     __syncthreads();
   }
 }
@@ -703,7 +717,19 @@ __global__ void __launch_bounds__(256) final_reduce_batch_kernel(const BatchRedu
 // contiguous arena): blockIdx.y is the entry
 __global__ void __launch_bounds__(256) copy_batch_kernel(const BatchCopy* __restrict__ tab) {
   const BatchCopy e = tab[blockIdx.y];
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < e.n; i += (int64_t)gridDim.x * 256) e.dst[i] = e.src[i];
+  const uint32_t* src = static_cast<const uint32_t*>(e.src);
+  uint32_t* dst = static_cast<uint32_t*>(e.dst);
+  if (((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 7) == 0) {
+    // both 8-byte aligned: pairs of words, then an odd last word
+    const uint2* s2 = reinterpret_cast<const uint2*>(src);
+    uint2* d2 = reinterpret_cast<uint2*>(dst);
+    const int64_t n2 = e.words / 2;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += (int64_t)gridDim.x * 256) d2[i] = s2[i];
+    if ((e.words & 1) && blockIdx.x == 0 && threadIdx.x == 0) dst[e.words - 1] = src[e.words - 1];
+  } else {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < e.words; i += (int64_t)gridDim.x * 256)
+      dst[i] = src[i];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -730,10 +756,11 @@ __global__ void rate_kernel(int het, const double* f0, const double* f, double d
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 9 * N) out[t] = het ? (f[t] - f0[t]) / dt : av[t / N];
 }
-// y = a*x + b*y0 (general axpby into y)
-__global__ void axpby_kernel(double a, const double* x, double b, const double* y0, double* y, int64_t n) {
+// y = a*x + b*y0 (general axpby into y; a float y0 enters as its double value)
+template <typename TY>
+__global__ void axpby_kernel(double a, const double* x, double b, const TY* y0, double* y, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) y[t] = a * x[t] + b * y0[t];
+  if (t < n) y[t] = a * x[t] + b * (double)y0[t];
 }
 // XA = XM*(1 - alpha_total) + sum_i beta_i Xdot_i   (VecScale + VecMAXPY)
 struct CombArgs {
@@ -1278,25 +1305,39 @@ void launch_partial_reduce(const int* ops, int64_t n, int nvec, const T* const* 
   hipLaunchKernelGGL(partial_reduce_kernel<T>, dim3(RED_BLOCKS, nvec), dim3(256), 0, st, r, partial);
   SCA_HIP(hipGetLastError());
 }
-void launch_shared_reduce(int mode, int64_t n, const double* u, const double* w, int nv, const double* const* v,
-                          double* partial, hipStream_t st) {
+template <typename T>
+void launch_shared_reduce(int mode, int64_t n, const T* u, const T* w, int nv, const T* const* v, double* partial,
+                          hipStream_t st) {
   if (mode != 0 && mode != 1) throw std::runtime_error("shared reduction mode");
   const int no = mode == 0 ? 2 : 1;
   for (int g0 = 0; g0 < nv; g0 += SHARED_GROUP) {
-    SharedRedArgs r{};
+    SharedRedArgs<T> r{};
     r.n = n;
     r.u = u;
     r.w = mode == 0 ? w : u;
     r.nv = std::min(SHARED_GROUP, nv - g0);
     for (int i = 0; i < SHARED_GROUP; ++i) r.v[i] = i < r.nv ? v[g0 + i] : u;  // unused slots stay dereferenceable
     double* out = partial + (int64_t)no * g0 * RED_BLOCKS;
-    if (mode == 0)
-      hipLaunchKernelGGL(shared_reduce_kernel<0>, dim3(RED_BLOCKS), dim3(256), 0, st, r, out);
-    else
-      hipLaunchKernelGGL(shared_reduce_kernel<1>, dim3(RED_BLOCKS), dim3(256), 0, st, r, out);
+    switch (mode * 100 + r.nv) {
+#define SCA_SR(NVv)                                                                                              \
+  case NVv:                                                                                                      \
+    hipLaunchKernelGGL((shared_reduce_kernel<0, T, NVv>), dim3(RED_BLOCKS), dim3(256), 0, st, r, out);         \
+    break;                                                                                                       \
+  case 100 + NVv:                                                                                                \
+    hipLaunchKernelGGL((shared_reduce_kernel<1, T, NVv>), dim3(RED_BLOCKS), dim3(256), 0, st, r, out);         \
+    break;
+      SCA_SR(1) SCA_SR(2) SCA_SR(3) SCA_SR(4) SCA_SR(5) SCA_SR(6) SCA_SR(7) SCA_SR(8)
+      SCA_SR(9) SCA_SR(10) SCA_SR(11) SCA_SR(12) SCA_SR(13) SCA_SR(14) SCA_SR(15) SCA_SR(16)
+#undef SCA_SR
+      default: throw std::runtime_error("shared reduction size");
+    }
     SCA_HIP(hipGetLastError());
   }
 }
+template void launch_shared_reduce<double>(int, int64_t, const double*, const double*, int, const double* const*,
+                                           double*, hipStream_t);
+template void launch_shared_reduce<float>(int, int64_t, const float*, const float*, int, const float* const*,
+                                          double*, hipStream_t);
 
 template void launch_partial_reduce<double>(const int*, int64_t, int, const double* const*, const double* const*,
                                             double*, hipStream_t);
@@ -1338,6 +1379,67 @@ __global__ void __launch_bounds__(256) dpdf_norm_partial(const double* d, int64_
     pv[2 * blockIdx.x + 1] = vmn[0];
   }
 }
+// dpdf_norm_partial fused with the sums of the 81 planes (C_volAvg): dPdF is
+// read once.  The per-thread element order and the block trees are those of
+// dpdf_norm_partial and of partial_reduce_kernel (op 2), so n2, the block
+// extremes and sums[q * RED_BLOCKS + b] are bitwise theirs.
+__global__ void __launch_bounds__(256) dpdf_reduce_partial(const double* d, int64_t N, double* n2, double* pv,
+                                                           double* sums) {
+  __shared__ double red[9][256];
+  double acc[81];
+#pragma unroll
+  for (int q = 0; q < 81; ++q) acc[q] = 0.0;
+  double bmx = -1.0, bmn = 1.7976931348623157e308;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < N; c += (int64_t)RED_BLOCKS * 256) {
+    double s = 0.0;
+#pragma unroll
+    for (int q0 = 0; q0 < 81; q0 += 9) {
+      double x[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) x[j] = d[(q0 + j) * N + c];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        s += x[j] * x[j];
+        acc[q0 + j] += x[j];
+      }
+    }
+    n2[c] = s;
+    bmx = fmax(bmx, s);
+    bmn = fmin(bmn, s);
+  }
+  const int t = threadIdx.x;
+  red[0][t] = bmx;
+  red[1][t] = bmn;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      red[0][t] = fmax(red[0][t], red[0][t + w]);
+      red[1][t] = fmin(red[1][t], red[1][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    pv[2 * blockIdx.x] = red[0][0];
+    pv[2 * blockIdx.x + 1] = red[1][0];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q0 = 0; q0 < 81; q0 += 9) {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) red[j][t] = acc[q0 + j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) red[j][t] += red[j][t + w];
+      }
+      __syncthreads();
+    }
+    if (t < 9) sums[(int64_t)(q0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
+    __syncthreads();
+  }
+}
+
 // Pass 2: per block the first cell with n2 >= hi and the first with n2 <= lo (-1: none)
 __global__ void __launch_bounds__(256) first_index_partial(const double* n2, int64_t N, double hi, double lo,
                                                            long long* pi) {
@@ -1366,9 +1468,12 @@ __global__ void __launch_bounds__(256) first_index_partial(const double* n2, int
   }
 }
 void tangent_minmax_select(const double* dPdF, int64_t N, double* n2, double* pv, long long* pi, int64_t& imax,
-                           int64_t& imin, hipStream_t st) {
+                           int64_t& imin, hipStream_t st, double* sums) {
   const int nb = RED_BLOCKS;
-  hipLaunchKernelGGL(dpdf_norm_partial, dim3(nb), dim3(256), 0, st, dPdF, N, n2, pv);
+  if (sums)
+    hipLaunchKernelGGL(dpdf_reduce_partial, dim3(nb), dim3(256), 0, st, dPdF, N, n2, pv, sums);
+  else
+    hipLaunchKernelGGL(dpdf_norm_partial, dim3(nb), dim3(256), 0, st, dPdF, N, n2, pv);
   SCA_HIP(hipGetLastError());
   std::vector<double> h(2 * nb);
   SCA_HIP(hipMemcpyAsync(h.data(), pv, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1506,7 +1611,7 @@ void launch_copy_batch(const BatchCopy* items, int n, TableArena& ta, hipStream_
   int64_t mx = 0;
   for (int i = 0; i < n; ++i) {
     h[i] = items[i];
-    mx = std::max(mx, items[i].n);
+    mx = std::max(mx, items[i].words / 2 + 1);
   }
   ta.upload(h, n, st);
   const unsigned bx = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (mx + 255) / 256), 1024);
@@ -1573,7 +1678,11 @@ void launch_rate(int het, const double* f0, const double* f, double dt, const do
   SCA_HIP(hipGetLastError());
 }
 void launch_axpby(double a, const double* x, double b, const double* y0, double* y, int64_t n, hipStream_t st) {
-  hipLaunchKernelGGL(axpby_kernel, dim3(nblk(n)), dim3(256), 0, st, a, x, b, y0, y, n);
+  hipLaunchKernelGGL(axpby_kernel<double>, dim3(nblk(n)), dim3(256), 0, st, a, x, b, y0, y, n);
+  SCA_HIP(hipGetLastError());
+}
+void launch_axpby(double a, const double* x, double b, const float* y0, double* y, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(axpby_kernel<float>, dim3(nblk(n)), dim3(256), 0, st, a, x, b, y0, y, n);
   SCA_HIP(hipGetLastError());
 }
 void launch_lincomb(const double* XM, double c0, int l, const double* beta, const double* const* X, double* XA,
