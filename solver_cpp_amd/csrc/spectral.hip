@@ -791,39 +791,56 @@ struct CombNormArgs {
   double beta[SHARED_GROUP];
   const double* X[SHARED_GROUP];
 };
+// L: history vectors in the launch (loads without branches); U elements per
+// trip, each computed as alone (the same results for any U)
+template <int L>
 __global__ void __launch_bounds__(256) lincomb_norms_kernel(CombNormArgs r, double* partial) {
   constexpr int CH = 8;
+  constexpr int NOUT = L + 1;
+  constexpr int U = L <= 4 ? 4 : 2;
   __shared__ double red[CH][256];
   const int t = threadIdx.x;
-  double s[SHARED_GROUP];
+  double s[NOUT];
 #pragma unroll
-  for (int j = 0; j < SHARED_GROUP; ++j) s[j] = 0.0;
+  for (int j = 0; j < NOUT; ++j) s[j] = 0.0;
   constexpr int64_t STRIDE = (int64_t)RED_BLOCKS * 256;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + t; e < r.n; e += STRIDE) {
-    const double xm = r.XM[e];
-    double h[SHARED_GROUP - 1];
-#pragma unroll
-    for (int i = 0; i < SHARED_GROUP - 1; ++i) h[i] = (i < r.l) ? r.X[i][e] : 0.0;
+  auto one = [&](double xm, const double* h, int64_t e) {
+    // the product xm c0 rounded on its own, then one fused multiply-add per
+    // history vector (as in launch_lincomb, whatever L lets the compiler see)
     double v = xm * r.c0;
 #pragma unroll
-    for (int i = 0; i < SHARED_GROUP - 1; ++i)
-      if (i < r.l) v += r.beta[i] * h[i];
+    for (int i = 0; i < L; ++i) v = __builtin_fma(r.beta[i], h[i], v);
     r.XA[e] = v;
     const double d0 = xm - v;
     s[0] += d0 * d0;
 #pragma unroll
-    for (int i = 0; i < SHARED_GROUP - 1; ++i)
-      if (i < r.l) {
-        const double d = h[i] - v;
-        s[1 + i] += d * d;
-      }
+    for (int i = 0; i < L; ++i) {
+      const double d = h[i] - v;
+      s[1 + i] += d * d;
+    }
+  };
+  int64_t e = (int64_t)blockIdx.x * 256 + t;
+  for (; e + (U - 1) * STRIDE < r.n; e += U * STRIDE) {
+    double xm[U], h[U][L];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xm[u] = r.XM[e + u * STRIDE];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int i = 0; i < L; ++i) h[u][i] = r.X[i][e + u * STRIDE];
+#pragma unroll
+    for (int u = 0; u < U; ++u) one(xm[u], h[u], e + u * STRIDE);
   }
-  const int nout = r.l + 1;
+  for (; e < r.n; e += STRIDE) {
+    double h[L];
 #pragma unroll
-  for (int c0 = 0; c0 < SHARED_GROUP; c0 += CH) {
-    if (c0 >= nout) break;
+    for (int i = 0; i < L; ++i) h[i] = r.X[i][e];
+    one(r.XM[e], h, e);
+  }
 #pragma unroll
-    for (int k = 0; k < CH; ++k) red[k][t] = s[c0 + k];
+  for (int c0 = 0; c0 < NOUT; c0 += CH) {
+#pragma unroll
+    for (int k = 0; k < CH; ++k) red[k][t] = c0 + k < NOUT ? s[c0 + k] : 0.0;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
       if (t < w) {
@@ -832,7 +849,7 @@ __global__ void __launch_bounds__(256) lincomb_norms_kernel(CombNormArgs r, doub
       }
       __syncthreads();
     }
-    if (t < CH && c0 + t < nout) partial[(int64_t)(c0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
+    if (t < CH && c0 + t < NOUT) partial[(int64_t)(c0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
     __syncthreads();
   }
 }
@@ -1711,7 +1728,14 @@ bool launch_lincomb_norms(const double* XM, double c0, int l, const double* beta
     a.beta[i] = i < l ? beta[i] : 0.0;
     a.X[i] = i < l ? X[i] : XM;  // unused slots stay dereferenceable
   }
-  hipLaunchKernelGGL(lincomb_norms_kernel, dim3(RED_BLOCKS), dim3(256), 0, st, a, partial);
+  switch (l) {
+#define SCA_LC(Lv)                                                                                   \
+  case Lv: hipLaunchKernelGGL(lincomb_norms_kernel<Lv>, dim3(RED_BLOCKS), dim3(256), 0, st, a, partial); break;
+    SCA_LC(1) SCA_LC(2) SCA_LC(3) SCA_LC(4) SCA_LC(5) SCA_LC(6) SCA_LC(7) SCA_LC(8)
+    SCA_LC(9) SCA_LC(10) SCA_LC(11) SCA_LC(12) SCA_LC(13) SCA_LC(14) SCA_LC(15)
+#undef SCA_LC
+    default: return false;
+  }
   SCA_HIP(hipGetLastError());
   return true;
 }
