@@ -462,6 +462,57 @@ def test_mixed_precision_16cubed(product, oracle):
     close(a.field("P"), b.field("P"), 1e-4, "P")
 
 
+def test_mixed_precision_128cubed(product, oracle):
+    """The mixed-precision configuration (128^3 grid, 1000 Voronoi grains,
+    tension to 3 % in 60 increments).  Anchored to the oracle at its own size
+    by one residual evaluation after init and forward (P and r within 1e-10;
+    the oracle's share is one 128^3 evaluation), then the first six
+    increments of the mixed form against the GPU's FP64 form (itself held to
+    the oracle by the 16^3 and 64^3 load-case tests): <P> within 2e-5 and
+    every cell's P within 1e-4 of max |P| (both forms stop anywhere within
+    eps_rel_div 5e-4), and with both at 1000-fold tighter tolerances over
+    the first two increments <P> within 1e-6 (the error of precision
+    itself)."""
+    cfg = CONFIGS["C3"]
+    e = cfg["edge"]
+    mat, ori, size = polycrystal((e, e, e), cfg["grains"], seed=cfg["seed"], ori_seed=cfg["ori_seed"])
+    load = [tension_x(t=cfg["t"], N=cfg["N"])]
+    a = Rve(product, (e, e, e), size, mat, ori, **EXAMPLE_NUMERICS)
+    b = Rve(oracle, (e, e, e), size, mat, ori, **EXAMPLE_NUMERICS)
+    a.init()
+    b.init()
+    dt = cfg["t"] / cfg["N"]
+    a.forward(False, False, dt, 1.0, cfg["t"], load[0])
+    b.forward(False, False, dt, 1.0, cfg["t"], load[0])
+    F = b.field("F")
+    ra, rb = a.form_residual(F), b.form_residual(F)
+    close(ra, rb, REL, "r (128^3)")
+    close(a.field("P"), b.field("P"), REL, "P (128^3)")
+    a.close()
+    b.close()
+    del b
+
+    def run(precision, n, **num):
+        r = Rve(product, (e, e, e), size, mat, ori, precision=precision, **dict(EXAMPLE_NUMERICS, **num))
+        r.init()
+        for d in r.increments(load):
+            if d["increment"] == n:
+                break
+        out = (np.array(r.scalars().P_av), r.field("P"), r.stats())
+        r.close()
+        return out
+
+    pm, Pm, st = run(1, 6)
+    pf, Pf, _ = run(0, 6)
+    assert st.fp32_evaluations > 0
+    close(pm, pf, 2e-5, "<P> mixed vs FP64, 6 increments")
+    close(Pm, Pf, 1e-4, "P mixed vs FP64, 6 increments")
+    tight = dict(itmax=100, eps_rel_div=5e-7, eps_abs_div=1e-12, eps_rel_P=1e-6, eps_abs_P=1e-6)
+    pm, _, _ = run(1, 2, **tight)
+    pf, _, _ = run(0, 2, **tight)
+    close(pm, pf, 1e-6, "<P> mixed vs FP64, tight tolerances")
+
+
 def test_mixed_precision_tight_tolerances(product, oracle):
     """Precision error apart from the solvers' stopping spread: with the
     divergence and stress tolerances 1000 times tighter (eps_rel_div 5e-7,
