@@ -193,7 +193,8 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
     const bool measure = reqs[idx[0]]->rve->desc.grid_num.fft_plan_mode == 1;
     const int64_t N = (int64_t)g.cells[0] * g.cells[1] * g.cells[2];
     const int64_t K = (int64_t)(g.cells[0] / 2 + 1) * g.cells[1] * g.cells[2];
-    // the strided passes launch 9 x (RVEs) x (other length) block rows
+    // the generic strided passes launch 9 x (RVEs) x (other length) block
+    // rows, at most 65535
     const int ymax = std::max(g.cells[1], g.cells[2]);
     const int chunk = std::max(1, std::min((int)idx.size(), 65535 / (9 * ymax)));
     ensure(g, chunk);

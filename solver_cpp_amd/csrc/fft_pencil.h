@@ -213,18 +213,22 @@ __device__ __forceinline__ const cpx<T>* stage_tw(cpx<T>* lds_tw, const cpx<T>* 
 // (o / d1) * s1 + (o % d1) * s2 + i with element stride es; a block takes P
 // consecutive i of one o, the pencil index fastest across lanes
 template <typename T, int N, int SIGN>
-__global__ void __launch_bounds__(MAX_PENCIL_BLOCK) strided_kernel(cpx<T>* data, int ni, int d1, int64_t s1,
-                                                                   int64_t s2, int64_t es, const cpx<T>* tw_g) {
+__global__ void __launch_bounds__(MAX_PENCIL_BLOCK) strided_kernel(cpx<T>* data, int ni, int64_t npen, int d1,
+                                                                   int64_t s1, int64_t s2, int64_t es,
+                                                                   const cpx<T>* tw_g) {
   using S = Shape<T, N>;
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
   const int P = blockDim.x / S::TP;  // a power of two
   const cpx<T>* tw = stage_tw<T, N>(lds + P * S::LD, tw_g);
   const int p = threadIdx.x & (P - 1), t = threadIdx.x >> __builtin_ctz(P);
-  const int i = blockIdx.x * P + p;
-  const int64_t o = blockIdx.y;
+  // pencils (o, i) numbered with i fastest across the whole launch, so that a
+  // block does not hold a mostly empty row of i (ni = N/2 + 1 is odd)
+  const int64_t pen = (int64_t)blockIdx.x * P + p;
+  const int64_t o = pen / ni;
+  const int i = (int)(pen - o * ni);
   const int64_t base = (o / d1) * s1 + (o % d1) * s2 + i;
-  const bool ok = i < ni;
+  const bool ok = pen < npen;
   cpx<T> v[S::E];
 #pragma unroll
   for (int j = 0; j < S::E; ++j) v[j] = ok ? data[base + (int64_t)(t + S::TP * j) * es] : cpx<T>{(T)0, (T)0};

@@ -1000,13 +1000,14 @@ static bool pencil_strided(int n, int sign, cpx<T>* data, int ni, unsigned no, i
   case N: {                                                                                                  \
     using S = pencil::Shape<T, N>;                                                                              \
     const int P = bs / S::TP;                                                                                \
-    const dim3 grid((unsigned)((ni + P - 1) / P), no);                                                      \
+    const int64_t npen = (int64_t)ni * no;                                                                   \
+    const dim3 grid((unsigned)((npen + P - 1) / P));                                                         \
     if (sign < 0)                                                                                            \
       hipLaunchKernelGGL((pencil::strided_kernel<T, N, -1>), grid, dim3(bs), (pencil_lds<T, N>(bs)), st, data, \
-                         ni, d1, s1, s2, es, tw);                                                            \
+                         ni, npen, d1, s1, s2, es, tw);                                                      \
     else                                                                                                     \
       hipLaunchKernelGGL((pencil::strided_kernel<T, N, 1>), grid, dim3(bs), (pencil_lds<T, N>(bs)), st, data,  \
-                         ni, d1, s1, s2, es, tw);                                                            \
+                         ni, npen, d1, s1, s2, es, tw);                                                      \
     return true;                                                                                             \
   }
     SCA_PENCIL_LENGTHS(SCA_CASE)
