@@ -45,6 +45,51 @@ __device__ constexpr int HTYPE[NSLIP][NSLIP] = {
 __device__ __forceinline__ constexpr int kcoefA(int s, int m, int l) { return psg(s, m, l) + psg(s, l, m); }
 __device__ __forceinline__ constexpr int kcoefB(int s, int m, int l) { return m == l ? psg(s, m, m) : 0; }
 
+// Structure of the table used to share work between slip systems:
+// - the systems 3p, 3p+1, 3p+2 lie on the same plane (normal), p = 0..3;
+// - the systems g, g+3, g+6, g+9 (g = 0, 1, 2) have direction component g
+//   zero.
+// The Jacobian (assemble_jacobian8) needs, for every row r = 3i+j < 8 and
+// every symmetric pair (m,l), the sum over s of f_s P_s(i,j) k_s(m,l) with
+// k_s(m,l) = P_s(m,m) (m = l) or P_s(m,l) + P_s(l,m).  Restricted to one
+// direction group its coefficient vector is zero or plus/minus one of the
+// four Hadamard rows (1,1,1,1), (1,1,-1,-1), (1,-1,1,-1), (1,-1,-1,1), so
+// the sums come from the four Hadamard sums of each group's f_s.
+__device__ __forceinline__ constexpr int qcoef(int s, int r, int m, int l) {
+  return psg(s, r / 3, r % 3) * (m == l ? psg(s, m, m) : psg(s, m, l) + psg(s, l, m));
+}
+// 0 when group g does not enter Q_r(m,l), else sign * (1 + Hadamard row)
+__device__ __forceinline__ constexpr int qhad(int g, int r, int m, int l) {
+  const int a = qcoef(g, r, m, l), b = qcoef(g + 3, r, m, l), c = qcoef(g + 6, r, m, l);
+  if (a == 0) return 0;
+  return a * (1 + (b * a > 0 ? (c * a > 0 ? 0 : 1) : (c * a > 0 ? 2 : 3)));
+}
+constexpr bool slip_structure_holds() {
+  for (int p = 0; p < 4; ++p)
+    for (int k = 1; k < 3; ++k)
+      for (int j = 0; j < 3; ++j)
+        if (SLIP[3 * p + k][3 + j] != SLIP[3 * p][3 + j]) return false;
+  for (int s = 0; s < NSLIP; ++s)
+    if (SLIP[s][s % 3] != 0) return false;
+  const int had[4][4] = {{1, 1, 1, 1}, {1, 1, -1, -1}, {1, -1, 1, -1}, {1, -1, -1, 1}};
+  for (int r = 0; r < 8; ++r)
+    for (int m = 0; m < 3; ++m)
+      for (int l = m; l < 3; ++l) {
+        int groups = 0;
+        for (int g = 0; g < 3; ++g) {
+          const int code = qhad(g, r, m, l);
+          for (int k = 0; k < 4; ++k) {
+            const int want = code == 0 ? 0 : (code > 0 ? 1 : -1) * had[(code > 0 ? code : -code) - 1][k];
+            if (qcoef(g + 3 * k, r, m, l) != want) return false;
+          }
+          groups += code != 0;
+        }
+        if (groups == 0) return false;
+      }
+  return true;
+}
+static_assert(slip_structure_holds(), "cF slip table: planes of three systems, Hadamard-structured Jacobian sums");
+
 // Material constants travel in the kernel arguments (not a __constant__
 // symbol) so that launches of different contexts on concurrent streams never
 // share them.
@@ -163,15 +208,65 @@ __device__ __forceinline__ void hooke_S(const double* Fe, double* S, const MatCo
   S[6] = s13; S[7] = s23; S[8] = s33;
 }
 
-// tau_s = Mp : P_s with Mp = S (Fi = I)
-__device__ __forceinline__ double tau_of(const double* S, int s, double v) {
-  double t = 0.0;
+// tau_s = Mp : P_s with Mp = S (Fi = I), for all systems: tau_s = v d_s . (S n_s),
+// the three systems of a plane sharing S n (signed sums: every entry of d_s and
+// n_s is 0 or +-1)
+__device__ __forceinline__ void resolved_shears(const double* S, double v, double* tau) {
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
+  for (int p = 0; p < 4; ++p) {
+    double u[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      if (psg(s, i, j) != 0) t += S[3 * i + j] * (v * (double)psg(s, i, j));
-  return t;
+    for (int i = 0; i < 3; ++i) {
+      double t = SLIP[3 * p][3] > 0 ? S[3 * i] : -S[3 * i];
+#pragma unroll
+      for (int j = 1; j < 3; ++j) t = SLIP[3 * p][3 + j] > 0 ? t + S[3 * i + j] : t - S[3 * i + j];
+      u[i] = t;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int s = 3 * p + k;
+      double t = 0.0;
+      bool first = true;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (SLIP[s][i] != 0) {
+          t = first ? (SLIP[s][i] > 0 ? u[i] : -u[i]) : (SLIP[s][i] > 0 ? t + u[i] : t - u[i]);
+          first = false;
+        }
+      tau[s] = v * t;
+    }
+  }
+}
+
+// Lp = v sum_s g_s d_s n_s^T, the systems of a plane summed first:
+// Lp = v sum_p (sum_{s on p} g_s d_s) n_p^T (row-major 3x3 out)
+__device__ __forceinline__ void slip_sum(const double* g, double v, double* L) {
+  double w[4][3];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double t = 0.0;
+      bool first = true;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int s = 3 * p + k;
+        if (SLIP[s][i] != 0) {
+          t = first ? (SLIP[s][i] > 0 ? g[s] : -g[s]) : (SLIP[s][i] > 0 ? t + g[s] : t - g[s]);
+          first = false;
+        }
+      }
+      w[p][i] = t;
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double t = SLIP[0][3 + j] > 0 ? w[0][i] : -w[0][i];
+#pragma unroll
+      for (int p = 1; p < 4; ++p) t = SLIP[3 * p][3 + j] > 0 ? t + w[p][i] : t - w[p][i];
+      L[3 * i + j] = v * t;
+    }
 }
 
 // kinetics_sl (phenopowerlaw.f90:444-483): tau_pos = t, tau_neg = -t, taking
@@ -184,9 +279,11 @@ template <int NE4>
 __device__ __forceinline__ void kinetics(const double* S, const double* rxi, double* dg, double* c,
                                          const MatConst& mc) {
   constexpr double TINY10 = 2.2250738585072014e-308 * 10.0;
+  double tau[NSLIP];
+  resolved_shears(S, mc.v, tau);
 #pragma unroll
   for (int s = 0; s < NSLIP; ++s) {
-    const double tp = tau_of(S, s, mc.v);
+    const double tp = tau[s];
     const double tn = -tp;
     const double sgn = (tp > tn) ? 1.0 : -1.0;
     const double tmax = fmax(tp, tn);
@@ -292,42 +389,6 @@ __device__ __forceinline__ bool solve8_pivot(double* J, double* b) {
   return true;
 }
 
-// LU without row exchanges; refuses (returns false, J and b then hold no
-// meaning) when a pivot is smaller than 1e-6 of its column's largest entry.
-// Multipliers up to 1e6 cost at most a few of the 16 digits of the Newton
-// step, far below the Newton tolerance, and the residual test of the next
-// iteration uses the exact residual; only steeper cases (about 1 % of points
-// had pivots below 1/100 of their column in the 128^3 tension case) need the
-// pivoted pass.  The reciprocal pivots are kept on the diagonal for the back
-// substitution.
-__device__ __forceinline__ bool solve8_nopivot(double* J, double* b) {
-#pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    double mx = 0.0;
-#pragma unroll
-    for (int i = k + 1; i < NR; ++i) mx = fmax(mx, fabs(J[NR * i + k]));
-    const double piv = J[NR * k + k];
-    if (!(fabs(piv) * 1e6 >= mx) || piv == 0.0) return false;
-    const double inv = 1.0 / piv;
-    J[NR * k + k] = inv;
-#pragma unroll
-    for (int i = k + 1; i < NR; ++i) {
-      const double l = J[NR * i + k] * inv;
-#pragma unroll
-      for (int c = k + 1; c < NR; ++c) J[NR * i + c] -= l * J[NR * k + c];
-      b[i] -= l * b[k];
-    }
-  }
-#pragma unroll
-  for (int i = NR - 1; i >= 0; --i) {
-    double s = b[i];
-#pragma unroll
-    for (int c = i + 1; c < NR; ++c) s -= J[NR * i + c] * b[c];
-    b[i] = s * J[NR * i + i];
-  }
-  return true;
-}
-
 // Column-major plane q = i+3j of a (3,3) field <-> row-major register 3*i+j
 __device__ __forceinline__ int rm(int q) { return 3 * (q % 3) + q / 3; }
 
@@ -390,45 +451,132 @@ __device__ __forceinline__ void load33_sel(bool sel, const double* a, const doub
 
 // J = I + dt sum_s c_s vec(P_s) vec(G^T K_s)^T with G = Fe^T A, restricted to
 // the traceless coordinates (rows 0..7; columns 0 and 4 minus column 8), as
-// an 8x8 row-major matrix
-__device__ __forceinline__ void assemble_jacobian8(const double* Fe, const double* A, const double* c, double dt,
-                                                   double* J, const MatConst& mc) {
-  double G[9];
+// an 8x8 row-major matrix.
+//
+// With f_s = dt v c_s and K_s(m,l) = kappa(m,l) k_s(m,l) (kappa = 2 kA + kB on
+// the diagonal, kA off it), row r = 3i+j is
+//   J(r,(p,l)) = delta + sum_m kappa(m,l) G(m,p) Q_r(m,l),
+//   Q_r(m,l) = sum_s f_s P_s(i,j) k_s(m,l)   (symmetric in m,l),
+// and every Q_r(m,l) is a signed sum of one or two of the twelve Hadamard sums
+// of the direction groups' f_s (qhad): 24 additions for all the Q_r instead of
+// a rank-one update of the 8x8 matrix per slip system.
+// Row-independent part of the Jacobian: kappa G and the Hadamard sums
+struct JacobianParts {
+  double Gd[9], Go[9];  // kappa G for the diagonal / off-diagonal (m,l)
+  double H[3][4];       // Hadamard sums of f_s over each direction group
+};
+__device__ __forceinline__ void jacobian_parts(const double* Fe, const double* A, const double* c, double dt,
+                                               JacobianParts& jp, const MatConst& mc) {
+  const double kA = mc.v * mc.C44, kB = mc.v * (mc.C11 - mc.C12 - 2.0 * mc.C44);
+  const double kD = 2.0 * kA + kB;
 #pragma unroll
   for (int m = 0; m < 3; ++m)
 #pragma unroll
-    for (int p = 0; p < 3; ++p) G[3 * m + p] = Fe[m] * A[p] + Fe[3 + m] * A[3 + p] + Fe[6 + m] * A[6 + p];
-  const double kA = mc.v * mc.C44, kB = mc.v * (mc.C11 - mc.C12 - 2.0 * mc.C44);
+    for (int p = 0; p < 3; ++p) {
+      const double g = Fe[m] * A[p] + Fe[3 + m] * A[3 + p] + Fe[6 + m] * A[6 + p];
+      jp.Gd[3 * m + p] = g * kD;
+      jp.Go[3 * m + p] = g * kA;
+    }
+  const double fv = dt * mc.v;
 #pragma unroll
-  for (int k = 0; k < NR * NR; ++k) J[k] = (k % (NR + 1) == 0) ? 1.0 : 0.0;
-#pragma unroll
-  for (int s = 0; s < NSLIP; ++s) {
-    double W[9];  // W_s = G^T K_s
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        double w = 0.0;
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-          if (kcoefA(s, m, l) != 0) w += G[3 * m + p] * (kA * (double)kcoefA(s, m, l));
-          if (kcoefB(s, m, l) != 0) w += G[3 * m + p] * (kB * (double)kcoefB(s, m, l));
-        }
-        W[3 * p + l] = w;
-      }
-    W[0] -= W[8];
-    W[4] -= W[8];
-    const double f = dt * c[s] * mc.v;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        if (3 * i + j < NR && psg(s, i, j) != 0) {
-          const double fs = f * (double)psg(s, i, j);
-#pragma unroll
-          for (int q = 0; q < NR; ++q) J[NR * (3 * i + j) + q] += fs * W[q];
-        }
+  for (int g = 0; g < 3; ++g) {
+    const double a = c[g] * fv, b = c[g + 3] * fv, x = c[g + 6] * fv, y = c[g + 9] * fv;
+    const double ab = a + b, amb = a - b, xy = x + y, xmy = x - y;
+    jp.H[g][0] = ab + xy;
+    jp.H[g][1] = ab - xy;
+    jp.H[g][2] = amb + xmy;
+    jp.H[g][3] = amb - xmy;
   }
+}
+// row r of the 8x8 Jacobian
+__device__ __forceinline__ void jacobian_row(const JacobianParts& jp, int r, double* row) {
+  double Q[3][3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int l = m; l < 3; ++l) {
+      double t = 0.0;
+      bool first = true;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) {
+        const int code = qhad(g, r, m, l);
+        if (code != 0) {
+          const double h = jp.H[g][(code > 0 ? code : -code) - 1];
+          t = first ? (code > 0 ? h : -h) : (code > 0 ? t + h : t - h);
+          first = false;
+        }
+      }
+      Q[m][l] = t;
+      Q[l][m] = t;
+    }
+  double Y[9];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      double y = (l == 0 ? jp.Gd : jp.Go)[p] * Q[0][l];
+      y += (l == 1 ? jp.Gd : jp.Go)[3 + p] * Q[1][l];
+      y += (l == 2 ? jp.Gd : jp.Go)[6 + p] * Q[2][l];
+      Y[3 * p + l] = y;
+    }
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const double y = (q == 0 || q == 4) ? Y[q] - Y[8] : Y[q];
+    row[q] = (q == r) ? 1.0 + y : y;
+  }
+}
+__device__ __forceinline__ void assemble_jacobian8(const double* Fe, const double* A, const double* c, double dt,
+                                                   double* J, const MatConst& mc) {
+  JacobianParts jp;
+  jacobian_parts(Fe, A, c, dt, jp, mc);
+#pragma unroll
+  for (int r = 0; r < NR; ++r) jacobian_row(jp, r, J + NR * r);
+}
+
+// The Newton system assembled and solved by LU without row exchanges, one row
+// at a time: row i is eliminated by the factor rows 0..i-1 as soon as it is
+// formed (the same operations on every entry, in the same order, as the
+// column-by-column elimination), so only the factor rows are held.  Refuses
+// (returns false; b then holds no meaning) when a multiplier exceeds 1e6 in
+// magnitude or a pivot is zero: multipliers up to 1e6 cost at most a few of
+// the 16 digits of the Newton step, far below the Newton tolerance, and the
+// residual test of the next iteration uses the exact residual; only steeper
+// cases (about 1 % of points had pivots below 1/100 of their column in the
+// 128^3 tension case) need the pivoted pass.
+__device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const double* A, const double* c,
+                                                        double dt, double* b, const MatConst& mc) {
+  JacobianParts jp;
+  jacobian_parts(Fe, A, c, dt, jp, mc);
+  double U[NR * NR];  // factor rows (upper part), reciprocal pivots on the diagonal
+  // refusals are collected and tested once at the end (what is computed
+  // after one is discarded): no branches inside the unrolled elimination
+  bool refused = false;
+#pragma unroll
+  for (int i = 0; i < NR; ++i) {
+    double row[NR];
+    jacobian_row(jp, i, row);
+#pragma unroll
+    for (int k = 0; k < i; ++k) {
+      const double l = row[k] * U[NR * k + k];
+      refused = refused || fabs(l) > 1e6;
+#pragma unroll
+      for (int c2 = k + 1; c2 < NR; ++c2) row[c2] -= l * U[NR * k + c2];
+      b[i] -= l * b[k];
+    }
+    const double piv = row[i];
+    refused = refused || !(fabs(piv) > 0.0);
+    U[NR * i + i] = 1.0 / piv;
+#pragma unroll
+    for (int c2 = i + 1; c2 < NR; ++c2) U[NR * i + c2] = row[c2];
+  }
+#pragma unroll
+  for (int i = NR - 1; i >= 0; --i) {
+    double s = b[i];
+#pragma unroll
+    for (int c2 = i + 1; c2 < NR; ++c2) s -= U[NR * i + c2] * b[c2];
+    b[i] = s * U[NR * i + i];
+  }
+  return !refused;
 }
 
 // Per-point output / scratch addressing
@@ -492,15 +640,7 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
     double dg[NSLIP], c[NSLIP];
     kinetics<NE4>(Sl, rxi, dg, c, mc);
     double res[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) res[k] = 0.0;
-#pragma unroll
-    for (int s = 0; s < NSLIP; ++s)
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-          if (psg(s, i, j) != 0) res[3 * i + j] = res[3 * i + j] + dg[s] * (mc.v * (double)psg(s, i, j));
+    slip_sum(dg, mc.v, res);
     // res holds Lp_constitutive here
     double n2Lpc = 0.0, n2Lp = 0.0, n2res = 0.0;
 #pragma unroll
@@ -533,11 +673,12 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
     {
       double J[NR * NR], A[9];
       loadA(A);
-      assemble_jacobian8(Fe, A, c, dt, J, mc);
       if constexpr (PIVOT) {
+        assemble_jacobian8(Fe, A, c, dt, J, mc);
         if (!solve8_pivot(J, res)) return 1;  // singular
       } else {
-        if (!solve8_nopivot(J, res)) return 2;  // the point is redone by the pivoting pass
+        // the point is redone by the pivoting pass
+        if (!assemble_solve8_nopivot(Fe, A, c, dt, res, mc)) return 2;
       }
     }
     res[8] = -(res[0] + res[4]);  // x_22 of the traceless solution
@@ -689,8 +830,10 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
   double stepFrac = 0.0, step = 1.0 / mc.r_cutback;
   bool converged = false, todo = true;
   bool refused = false;  // first pass only: the Newton system needs row exchanges
-  while (todo) {
-    double fp0v[9];  // Fp0 of the (sub)step about to be integrated (planes in order)
+  double fp0v[9];        // Fp0 of the (sub)step about to be integrated (planes in order)
+  // step control: the next substep after a converged one, else a cut-back
+  // step from the restored start values; sets todo
+  auto control = [&]() {
     if (converged) {
       stepFrac = stepFrac + step;
       step = fmin(1.0 - stepFrac, mc.r_increase * step);
@@ -704,6 +847,7 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
           fl[k] = ld(sub + (SUB_FLOC + k) * N, l);
           fp[k] = ld(s.Fp + (k) * N, l);
         }
+        __builtin_amdgcn_sched_barrier(0);  // all issued before the first store (one round trip)
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
           st(sub + (SUB_F0 + k) * N, l, fl[k]);
@@ -720,20 +864,33 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
       }
     } else {
       step = mc.r_cutback * step;
+      // every load of the restore is issued before the first of its values
+      // is stored (to LDS or memory): one memory round trip.  Left to
+      // itself, the scheduler interleaves them with the LDS stores of the
+      // shear column through a few staging registers, a round trip per pair.
+      double stl[NSTATE];
       load33(s.S0, N, l, S);
 #pragma unroll
       for (int k = 0; k < 9; ++k) fp0v[k] = ldv((in_sub ? sub + SUB_FP0 * N : s.Fp0) + k * N, l);
       if (step < 1.0) load33_sel(in_sub, sub + SUB_LP0 * N, s.Lp0, N, l, Lp);
 #pragma unroll
+      for (int k = 0; k < NSTATE; ++k) stl[k] = st0(k);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
       for (int k = 0; k < NSLIP; ++k) {
-        xi[k] = st0(k);
-        h_gam[k][t] = st0(NSLIP + k);
+        xi[k] = stl[k];
+        h_gam[k][t] = stl[NSLIP + k];
       }
 #pragma unroll
       for (int k = 0; k < 9; ++k) st(s.Fp + (k) * N, l, fp0v[k]);  // after the loads, as above
       todo = step > mc.r_cutback_min;
     }
-    if (todo) {
+  };
+  // the first call restores the increment's start values; it follows the
+  // entry loads directly, so that its loads and theirs share one round trip
+  control();
+  while (todo) {
+    {
       // A = F_(sub)step invFp0 and invFp0 are fixed during the state fixed
       // point (integrateStress, phase_mechanical.f90:425-432)
       bool fp0_singular;
@@ -866,6 +1023,7 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
       converged = !broken;
       if (refused) break;  // the pivoting pass redoes this point from its inputs
     }
+    control();
   }
   if constexpr (!PIVOT) {
     if (refused) {
