@@ -460,13 +460,17 @@ __device__ __forceinline__ void load33_sel(bool sel, const double* a, const doub
 // and every Q_r(m,l) is a signed sum of one or two of the twelve Hadamard sums
 // of the direction groups' f_s (qhad): 24 additions for all the Q_r instead of
 // a rank-one update of the 8x8 matrix per slip system.
-// Row-independent part of the Jacobian: kappa G and the Hadamard sums
+// Row-independent part of the Jacobian: kappa G (or, LEAN, G alone, kappa
+// then scaling Q in every row: 18 registers fewer for 48 more products) and
+// the Hadamard sums
+template <bool LEAN>
 struct JacobianParts {
-  double Gd[9], Go[9];  // kappa G for the diagonal / off-diagonal (m,l)
-  double H[3][4];       // Hadamard sums of f_s over each direction group
+  double Gd[9], Go[LEAN ? 1 : 9];  // kappa G for the diagonal / off-diagonal (m,l); LEAN: G in Gd
+  double H[3][4];                  // Hadamard sums of f_s over each direction group
 };
+template <bool LEAN>
 __device__ __forceinline__ void jacobian_parts(const double* Fe, const double* A, const double* c, double dt,
-                                               JacobianParts& jp, const MatConst& mc) {
+                                               JacobianParts<LEAN>& jp, const MatConst& mc) {
   const double kA = mc.v * mc.C44, kB = mc.v * (mc.C11 - mc.C12 - 2.0 * mc.C44);
   const double kD = 2.0 * kA + kB;
 #pragma unroll
@@ -474,8 +478,12 @@ __device__ __forceinline__ void jacobian_parts(const double* Fe, const double* A
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
       const double g = Fe[m] * A[p] + Fe[3 + m] * A[3 + p] + Fe[6 + m] * A[6 + p];
-      jp.Gd[3 * m + p] = g * kD;
-      jp.Go[3 * m + p] = g * kA;
+      if constexpr (LEAN) {
+        jp.Gd[3 * m + p] = g;
+      } else {
+        jp.Gd[3 * m + p] = g * kD;
+        jp.Go[3 * m + p] = g * kA;
+      }
     }
   const double fv = dt * mc.v;
 #pragma unroll
@@ -489,7 +497,8 @@ __device__ __forceinline__ void jacobian_parts(const double* Fe, const double* A
   }
 }
 // row r of the 8x8 Jacobian
-__device__ __forceinline__ void jacobian_row(const JacobianParts& jp, int r, double* row) {
+template <bool LEAN>
+__device__ __forceinline__ void jacobian_row(const JacobianParts<LEAN>& jp, int r, double* row, const MatConst& mc) {
   double Q[3][3];
 #pragma unroll
   for (int m = 0; m < 3; ++m)
@@ -506,6 +515,10 @@ __device__ __forceinline__ void jacobian_row(const JacobianParts& jp, int r, dou
           first = false;
         }
       }
+      if constexpr (LEAN) {
+        const double kA = mc.v * mc.C44, kB = mc.v * (mc.C11 - mc.C12 - 2.0 * mc.C44);
+        t = t * (m == l ? 2.0 * kA + kB : kA);
+      }
       Q[m][l] = t;
       Q[l][m] = t;
     }
@@ -514,10 +527,17 @@ __device__ __forceinline__ void jacobian_row(const JacobianParts& jp, int r, dou
   for (int p = 0; p < 3; ++p)
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
-      double y = (l == 0 ? jp.Gd : jp.Go)[p] * Q[0][l];
-      y += (l == 1 ? jp.Gd : jp.Go)[3 + p] * Q[1][l];
-      y += (l == 2 ? jp.Gd : jp.Go)[6 + p] * Q[2][l];
-      Y[3 * p + l] = y;
+      if constexpr (LEAN) {
+        double y = jp.Gd[p] * Q[0][l];
+        y += jp.Gd[3 + p] * Q[1][l];
+        y += jp.Gd[6 + p] * Q[2][l];
+        Y[3 * p + l] = y;
+      } else {
+        double y = (l == 0 ? jp.Gd : jp.Go)[p] * Q[0][l];
+        y += (l == 1 ? jp.Gd : jp.Go)[3 + p] * Q[1][l];
+        y += (l == 2 ? jp.Gd : jp.Go)[6 + p] * Q[2][l];
+        Y[3 * p + l] = y;
+      }
     }
 #pragma unroll
   for (int q = 0; q < NR; ++q) {
@@ -527,10 +547,10 @@ __device__ __forceinline__ void jacobian_row(const JacobianParts& jp, int r, dou
 }
 __device__ __forceinline__ void assemble_jacobian8(const double* Fe, const double* A, const double* c, double dt,
                                                    double* J, const MatConst& mc) {
-  JacobianParts jp;
+  JacobianParts<true> jp;
   jacobian_parts(Fe, A, c, dt, jp, mc);
 #pragma unroll
-  for (int r = 0; r < NR; ++r) jacobian_row(jp, r, J + NR * r);
+  for (int r = 0; r < NR; ++r) jacobian_row(jp, r, J + NR * r, mc);
 }
 
 // The Newton system assembled and solved by LU without row exchanges, one row
@@ -545,7 +565,7 @@ __device__ __forceinline__ void assemble_jacobian8(const double* Fe, const doubl
 // 128^3 tension case) need the pivoted pass.
 __device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const double* A, const double* c,
                                                         double dt, double* b, const MatConst& mc) {
-  JacobianParts jp;
+  JacobianParts<false> jp;
   jacobian_parts(Fe, A, c, dt, jp, mc);
   double U[NR * NR];  // factor rows (upper part), reciprocal pivots on the diagonal
   // refusals are collected and tested once at the end (what is computed
@@ -554,7 +574,7 @@ __device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const 
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
     double row[NR];
-    jacobian_row(jp, i, row);
+    jacobian_row(jp, i, row, mc);
 #pragma unroll
     for (int k = 0; k < i; ++k) {
       const double l = row[k] * U[NR * k + k];
@@ -886,10 +906,14 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
       todo = step > mc.r_cutback_min;
     }
   };
-  // the first call restores the increment's start values; it follows the
-  // entry loads directly, so that its loads and theirs share one round trip
-  control();
-  while (todo) {
+  // first pass: the first call restores the increment's start values; it
+  // follows the entry loads directly, so that its loads and theirs share one
+  // round trip.  The pivoting pass (a few points per launch, registers
+  // tighter) keeps the control at the top of the loop.
+  if constexpr (!PIVOT) control();
+  for (;;) {
+    if constexpr (PIVOT) control();
+    if (!todo) break;
     {
       // A = F_(sub)step invFp0 and invFp0 are fixed during the state fixed
       // point (integrateStress, phase_mechanical.f90:425-432)
@@ -1023,7 +1047,7 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
       converged = !broken;
       if (refused) break;  // the pivoting pass redoes this point from its inputs
     }
-    control();
+    if constexpr (!PIVOT) control();
   }
   if constexpr (!PIVOT) {
     if (refused) {
@@ -1202,12 +1226,20 @@ struct TangentArgs {
   MatConst mc;
   double dt;
   double* dPdF;
-  double C[81];  // C(i,j,k,l) at 27i+9j+3k+l (row-major)
   int64_t c_begin, c_end;
 };
 
-__device__ __forceinline__ double Cijkl(const TangentArgs& a, int i, int j, int k, int l) {
-  return a.C[27 * i + 9 * j + 3 * k + l];
+// (C : X)(i,j) = sum_kl C(i,j,k,l) X(k,l) for the cubic stiffness
+// (math_Voigt66to3333_stiffness of C11, C12, C44): C11 X_ii + C12 sum_{k!=i}
+// X_kk on the diagonal, C44 (X_ij + X_ji) off it (row-major 3x3)
+__device__ __forceinline__ void cubic_contract(const double* X, double* Y, const MatConst& mc) {
+  const double t = X[0] + X[4] + X[8];
+  const double d = mc.C11 - mc.C12;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      Y[3 * i + j] = (i == j) ? mc.C12 * t + d * X[4 * i] : mc.C44 * (X[3 * i + j] + X[3 * j + i]);
 }
 }  // namespace
 
@@ -1242,26 +1274,6 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
   for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
   kinetics<NE4>(S, rxi, dg, cs, mc);
 
-  // rhs(p,o,:,:) = Fe C(p,o,:,:) invFp^T  -> R[(p,o)][(k,l)]
-  double R[81];
-#pragma unroll
-  for (int p = 0; p < 3; ++p)
-#pragma unroll
-    for (int o = 0; o < 3; ++o) {
-      double FC[9];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-          FC[3 * k + q] = Fe[3 * k] * Cijkl(a, p, o, 0, q) + Fe[3 * k + 1] * Cijkl(a, p, o, 1, q) +
-                          Fe[3 * k + 2] * Cijkl(a, p, o, 2, q);
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int l = 0; l < 3; ++l)
-          R[9 * (3 * p + o) + 3 * k + l] =
-              FC[3 * k] * invFp[3 * l] + FC[3 * k + 1] * invFp[3 * l + 1] + FC[3 * k + 2] * invFp[3 * l + 2];
-    }
   // L = I + dt sum_s c_s vec(U_s) vec(P_s)^T, U_s = C : (H P_s), H = Fe^T F invFp0
   double T2[9], H[9];
   mm33(F, invFp0, T2);
@@ -1286,17 +1298,7 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
         Z[3 * i + l] = t;
       }
     double U[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        double t = 0.0;
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-          for (int l = 0; l < 3; ++l) t += Cijkl(a, i, j, m, l) * Z[3 * m + l];
-        U[3 * i + j] = t;
-      }
+    cubic_contract(Z, U, mc);
     const double f = dt * cs[sl];
 #pragma unroll
     for (int q = 0; q < 9; ++q)
@@ -1306,58 +1308,51 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
         for (int o = 0; o < 3; ++o)
           if (psg(sl, p, o) != 0) L[9 * q + 3 * p + o] += f * U[q] * (mc.v * (double)psg(sl, p, o));
   }
-  // dSdF = L^-1 R: LU of L then 9 right-hand sides
-  {
+  // dSdF = L^-1 R with R(:, (k,l)) = (Fe C(:,:,:,:) invFp^T)(k,l): L is
+  // factorised once (partial pivoting, as LAPACK's dgesv: first largest entry
+  // of the column; the multipliers stay where their step formed them, the row
+  // exchanged at step k in piv[k]), and each column of R is formed, solved and
+  // used in turn, so that R is never held whole.  Each entry of a column
+  // receives the same exchanges and eliminations, in the same order, as in
+  // the elimination of L and R together.
+  int piv[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      int p = k;
-      double mx = fabs(L[9 * k + k]);
+  for (int k = 0; k < 9; ++k) {
+    int p = k;
+    double mx = fabs(L[9 * k + k]);
 #pragma unroll
-      for (int i = k + 1; i < 9; ++i) {
-        const double v = fabs(L[9 * i + k]);
-        if (v > mx) {
-          mx = v;
-          p = i;
-        }
-      }
-#pragma unroll
-      for (int i = k + 1; i < 9; ++i) {
-        const bool sw = (p == i);
-#pragma unroll
-        for (int cc = 0; cc < 9; ++cc) {
-          const double x = L[9 * k + cc], y = L[9 * i + cc];
-          L[9 * k + cc] = sw ? y : x;
-          L[9 * i + cc] = sw ? x : y;
-          const double u = R[9 * k + cc], w = R[9 * i + cc];
-          R[9 * k + cc] = sw ? w : u;
-          R[9 * i + cc] = sw ? u : w;
-        }
-      }
-      const double inv = 1.0 / L[9 * k + k];
-#pragma unroll
-      for (int i = k + 1; i < 9; ++i) {
-        const double l = L[9 * i + k] * inv;
-#pragma unroll
-        for (int cc = k + 1; cc < 9; ++cc) L[9 * i + cc] -= l * L[9 * k + cc];
-#pragma unroll
-        for (int cc = 0; cc < 9; ++cc) R[9 * i + cc] -= l * R[9 * k + cc];
+    for (int i = k + 1; i < 9; ++i) {
+      const double v = fabs(L[9 * i + k]);
+      if (v > mx) {
+        mx = v;
+        p = i;
       }
     }
+    piv[k] = p;
 #pragma unroll
-    for (int i = 8; i >= 0; --i)
+    for (int i = k + 1; i < 9; ++i) {
+      const bool sw = (p == i);
 #pragma unroll
-      for (int cc = 0; cc < 9; ++cc) {
-        double t = R[9 * i + cc];
-#pragma unroll
-        for (int j = i + 1; j < 9; ++j) t -= L[9 * i + j] * R[9 * j + cc];
-        R[9 * i + cc] = t / L[9 * i + i];
+      for (int cc = k; cc < 9; ++cc) {
+        const double x = L[9 * k + cc], y = L[9 * i + cc];
+        L[9 * k + cc] = sw ? y : x;
+        L[9 * i + cc] = sw ? x : y;
       }
+    }
+    const double inv = 1.0 / L[9 * k + k];
+    L[9 * k + k] = inv;  // the reciprocal pivot, for the back substitution
+#pragma unroll
+    for (int i = k + 1; i < 9; ++i) {
+      const double l = L[9 * i + k] * inv;
+      L[9 * i + k] = l;
+#pragma unroll
+      for (int cc = k + 1; cc < 9; ++cc) L[9 * i + cc] -= l * L[9 * k + cc];
+    }
   }
-  // R now holds dSdF[(i,j)][(p,o)].
-  // dFpinvdF(:,:,p,o) = -dt sum_s c_s V_s(p,o) (invFp0 P_s) with
-  // V_s(p,o) = P_s : dSdF(:,:,p,o), formed column by column below (one
-  // column of dSdF at a time, so that no third 81-entry array is live)
-  // assemble dPdF
+  // dPdF = d(Fe S Fp^-T ...)/dF, column q = (p,o) of F at a time:
+  // dFpinvdF(:,:,p,o) = -dt sum_s c_s V_s(p,o) (invFp0 P_s)
+  //                   = invFp0 (sum_s w_s P_s),  w_s = -dt c_s V_s(p,o),
+  // V_s(p,o) = P_s : dSdF(:,:,p,o)
   double SiT[9], iFt[9], FiFp[9], u3[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -1372,40 +1367,49 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) d0[3 * i + j] = tmp[3 * j + i];
+  // a runtime loop over the rows p of F (three columns of R at a time): the
+  // three columns' solves are independent chains that hide each other's
+  // latency, and fewer columns in flight keep the registers within bounds
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
+  for (int p = 0; p < 3; ++p) {
+    double fe[3];  // row p of Fe (chosen by value: no register array indexed at run time)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) fe[m] = p == 0 ? Fe[m] : (p == 1 ? Fe[3 + m] : Fe[6 + m]);
 #pragma unroll
     for (int o = 0; o < 3; ++o) {
-      const int q = 3 * p + o;
-      double dF[9], dS[9];
+      // column (p,o) of R: (Fe C invFp^T)(i,j,p,o) = C : (Fe(p,:) (x) invFp(o,:))
+      double X[9], dS[9];
 #pragma unroll
-      for (int kl = 0; kl < 9; ++kl) {
-        dF[kl] = 0.0;
-        dS[kl] = R[9 * kl + q];
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) X[3 * m + r] = fe[m] * invFp[3 * o + r];
+      cubic_contract(X, dS, mc);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < 9; ++i) {
+          const bool sw = piv[k] == i;
+          const double x = dS[k], y = dS[i];
+          dS[k] = sw ? y : x;
+          dS[i] = sw ? x : y;
+        }
+#pragma unroll
+        for (int i = k + 1; i < 9; ++i) dS[i] -= L[9 * i + k] * dS[k];
       }
 #pragma unroll
-      for (int sl = 0; sl < NSLIP; ++sl) {
-        double V = 0.0;  // V_s(p,o)
+      for (int i = 8; i >= 0; --i) {
+        double t = dS[i];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-            if (psg(sl, i, j) != 0) V += (mc.v * (double)psg(sl, i, j)) * dS[3 * i + j];
-        double Q[9];  // invFp0 P_s
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-          for (int l = 0; l < 3; ++l) {
-            double t = 0.0;
-#pragma unroll
-            for (int m = 0; m < 3; ++m)
-              if (psg(sl, m, l) != 0) t += invFp0[3 * k + m] * (mc.v * (double)psg(sl, m, l));
-            Q[3 * k + l] = t;
-          }
-        const double f = -dt * cs[sl];
-#pragma unroll
-        for (int kl = 0; kl < 9; ++kl) dF[kl] += f * Q[kl] * V;
+        for (int j = i + 1; j < 9; ++j) t -= L[9 * i + j] * dS[j];
+        dS[i] = t * L[9 * i + i];
       }
+      // dS = dSdF(:,:,p,o)
+      double w[NSLIP], M[9], dF[9];
+      resolved_shears(dS, mc.v, w);
+#pragma unroll
+      for (int sl = 0; sl < NSLIP; ++sl) w[sl] = -dt * cs[sl] * w[sl];
+      slip_sum(w, mc.v, M);
+      mm33(invFp0, M, dF);
       double a1[9], a2[9], a3[9], t1[9], t2[9], dFt[9];
       mm33(F, dF, t1);
       mm33(t1, SiT, a1);
@@ -1426,6 +1430,7 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
           a.dPdF[(int64_t)(k + 3 * l + 9 * p + 27 * o) * N + c] = v;
         }
     }
+  }
 }
 
 // f_materialpoint_forward: state0 <- state (phase_mechanical.f90:992-1007)
@@ -1597,16 +1602,6 @@ void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* 
   a.dPdF = dPdF;
   a.c_begin = c_begin;
   a.c_end = c_end;
-  // cubic stiffness tensor from C11, C12, C44 (math_Voigt66to3333_stiffness)
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j)
-      for (int k = 0; k < 3; ++k)
-        for (int l = 0; l < 3; ++l) {
-          double v = 0.0;
-          if (i == j && k == l) v = (i == k) ? mc.C11 : mc.C12;
-          else if ((i == k && j == l) || (i == l && j == k)) v = mc.C44;
-          a.C[27 * i + 9 * j + 3 * k + l] = v;
-        }
   const int blocks = (int)((c_end - c_begin + 63) / 64);
   // the example material's slip-rate exponent (n_sl = 20) is compiled in:
   // the slip rates in the response kernel's form (x^(n-1) from one powering
