@@ -164,6 +164,31 @@ __device__ __forceinline__ double powq(double x, int e4, double e) {
   return r;
 }
 // compile-time quarter exponent E4/4 (E4 >= 0): fully unrolled
+// x^(-1/4) for x in [2^-1000, 2^1000]: a start value from the hardware
+// square root and reciprocal square root, then three Newton steps
+// y <- y (5 - x y^4) / 4 (quadratic: full precision from any start within
+// 2^-8 of the root); two correctly rounded square roots cost about three
+// times as many instructions
+__device__ __forceinline__ double rqrt_refined(double x) {
+  double y = __builtin_amdgcn_rsq(__builtin_amdgcn_sqrt(x));
+  const double mq = -0.25 * x;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double y2 = y * y;
+    y = y * __builtin_fma(mq, y2 * y2, 1.25);
+  }
+  return y;
+}
+// 1/x for the slip resistances (positive, far from the range limits): the
+// hardware reciprocal refined by two Newton steps, as the compiler's own
+// division does before its range fix-ups
+__device__ __forceinline__ double rcp_refined(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return r;
+}
+
 template <int E4>
 __device__ __forceinline__ double powc(double x) {
   double r = 1.0, b = x;
@@ -173,10 +198,17 @@ __device__ __forceinline__ double powc(double x) {
     b *= b;
   }
   constexpr int q = E4 & 3;
-  if (q) {
-    const double h = sqrt(x);
-    const double f = sqrt(h);
-    r *= (q == 1) ? f : (q == 2) ? h : h * f;
+  if constexpr (q == 2) {
+    r *= sqrt(x);
+  } else if constexpr (q != 0) {
+    // x^(1/4) = x y^3 and x^(3/4) = x y with y = x^(-1/4) (within 10 ulp),
+    // y taken at x clamped to [2^-1000, 2^1000] and multiplied by the true x:
+    // for x = 0 this gives 0, NaN propagates, and outside the range the
+    // power x^(E4 >> 2) >= x^1 (E4 >= 4) underflows to 0 or overflows to inf
+    // as the exact result does (E4 < 4 exponents run the runtime path)
+    static_assert(E4 >= 4, "quarter exponents below 1 are not compiled in");
+    const double y = rqrt_refined(fmin(fmax(x, 0x1p-1000), 0x1p1000));
+    r *= (q == 1) ? x * (y * (y * y)) : x * y;
   }
   return r;
 }
@@ -569,8 +601,11 @@ __device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const 
   jacobian_parts(Fe, A, c, dt, jp, mc);
   double U[NR * NR];  // factor rows (upper part), reciprocal pivots on the diagonal
   // refusals are collected and tested once at the end (what is computed
-  // after one is discarded): no branches inside the unrolled elimination
+  // after one is discarded): no branches inside the unrolled elimination, and
+  // the multipliers' magnitudes enter one running maximum (a NaN multiplier
+  // is ignored by the maximum, as by the comparison it replaces)
   bool refused = false;
+  double lmax = 0.0;
 #pragma unroll
   for (int i = 0; i < NR; ++i) {
     double row[NR];
@@ -578,7 +613,7 @@ __device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const 
 #pragma unroll
     for (int k = 0; k < i; ++k) {
       const double l = row[k] * U[NR * k + k];
-      refused = refused || fabs(l) > 1e6;
+      lmax = fmax(lmax, fabs(l));
 #pragma unroll
       for (int c2 = k + 1; c2 < NR; ++c2) row[c2] -= l * U[NR * k + c2];
       b[i] -= l * b[k];
@@ -596,7 +631,7 @@ __device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const 
     for (int c2 = i + 1; c2 < NR; ++c2) s -= U[NR * i + c2] * b[c2];
     b[i] = s * U[NR * i + i];
   }
-  return !refused;
+  return !(refused || lmax > 1e6);
 }
 
 // Per-point output / scratch addressing
@@ -671,13 +706,12 @@ __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const do
       n2res += res[k] * res[k];
     }
     {
-      // |res| < max(eps_rel max(|Lp|, |Lp_c|), eps_abs), compared squared
-      const double atol = fmax(mc.eps_rel_Lp * sqrt(fmax(n2Lp, n2Lpc)), mc.eps_abs_Lp);
-      bool nan = false;
-#pragma unroll
-      for (int k = 0; k < 9; ++k) nan = nan || isnan(res[k]);
-      if (nan) return 1;
-      if (n2res < atol * atol) break;
+      // |res| < max(eps_rel max(|Lp|, |Lp_c|), eps_abs), compared squared:
+      // |res|^2 < max(eps_rel^2 max(|Lp|^2, |Lp_c|^2), eps_abs^2), no square root
+      const double tol2 = fmax((mc.eps_rel_Lp * mc.eps_rel_Lp) * fmax(n2Lp, n2Lpc), mc.eps_abs_Lp * mc.eps_abs_Lp);
+      // a sum of squares is NaN exactly when one of its terms is
+      if (isnan(n2res)) return 1;
+      if (n2res < tol2) break;
       if (it == 1 || n2res < res_old_n2) {
         res_old_n2 = n2res;
 #pragma unroll
@@ -937,7 +971,7 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
       bool broken = true;
       double dot[NSTATE], rxi[NSLIP];
 #pragma unroll
-      for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
+      for (int k = 0; k < NSLIP; ++k) rxi[k] = rcp_refined(xi[k]);
       {
         double stt[NSTATE];
 #pragma unroll
@@ -975,7 +1009,7 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
             }
           }
 #pragma unroll
-          for (int k = 0; k < NSLIP; ++k) rxi[k] = 1.0 / xi[k];
+          for (int k = 0; k < NSLIP; ++k) rxi[k] = rcp_refined(xi[k]);
           const int rc =
               fp0_singular ? 1 : integrate_stress<NE4, PIVOT>(sh.A, sh.iFp0, t, sdt, rxi, Lp, S, newton, io, mc);
           refused = rc == 2;
