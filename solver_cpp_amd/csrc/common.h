@@ -148,6 +148,17 @@ int mp_sub_planes();
 void launch_pav_partial(const double* P, int64_t N, double* partial, hipStream_t st);
 void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF /*(81,N) SoA*/,
                        hipStream_t st, int64_t c_begin = 0, int64_t c_end = -1);
+// the consistent tangent of all N cells reduced instead of stored: n2 (N
+// doubles) the cells' |dPdF|^2, bpart (81 x tangent_partials(N) doubles) the
+// plane sums per 16 cells, at plane * tangent_partials(N) + cell / 16
+int64_t tangent_blocks(int64_t N);
+int64_t tangent_partials(int64_t N);
+void launch_mp_tangent_reduce(const MpState& s, const MatConst& mc, double dt, double* n2, double* bpart,
+                              hipStream_t st);
+// dPdF of n <= 64 listed cells (device array d_cells) into out (81 per cell,
+// plane k+3l+9p+27o), the values launch_mp_tangent gives for those cells
+void launch_mp_tangent_cells(const MpState& s, const MatConst& mc, double dt, const int64_t* d_cells, int n,
+                             double* out, hipStream_t st);
 void launch_mp_forward(const MpState& s, hipStream_t st);
 void launch_mp_init(const MpState& s, const double* d_Fp_init /*(9,N)*/, double xi0, hipStream_t st);
 

@@ -1259,8 +1259,12 @@ struct TangentArgs {
   MpState s;
   MatConst mc;
   double dt;
-  double* dPdF;
-  int64_t c_begin, c_end;
+  double* dPdF;  // MODE 0: 81 planes of N (plane k+3l+9p+27o); MODE 2: 81 values per listed cell
+  double* n2;    // MODE 1: per cell |dPdF|^2 (the 81 squares summed in the order they are formed)
+  double* bpart; // MODE 1: per 16 cells the sums of the 81 planes, at plane * nblk + 4 * block + row
+  int64_t nblk;  // MODE 1: 4 x the blocks of 64 cells
+  const int64_t* cells;    // MODE 2: the listed cells
+  int64_t c_begin, c_end;  // cells [c_begin, c_end); MODE 2: c_end - c_begin listed cells (at most 64)
 };
 
 // (C : X)(i,j) = sum_kl C(i,j,k,l) X(k,l) for the cubic stiffness
@@ -1277,14 +1281,50 @@ __device__ __forceinline__ void cubic_contract(const double* X, double* Y, const
 }
 }  // namespace
 
-// NE4: the slip-rate exponent n_sl as a compile-time quarter (-1: runtime)
-template <int NE4>
+// x[j] += x[j] of the lane DPP control CTRL selects, for nine doubles (the
+// halves moved separately; all lanes take part)
+template <int CTRL>
+__device__ __forceinline__ void dpp_step(double* x) {
+  int lo[9], hi[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) {
+    lo[j] = __builtin_amdgcn_update_dpp(0, __double2loint(x[j]), CTRL, 0xF, 0xF, false);
+    hi[j] = __builtin_amdgcn_update_dpp(0, __double2hiint(x[j]), CTRL, 0xF, 0xF, false);
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) x[j] += __hiloint2double(hi[j], lo[j]);
+}
+
+// NE4: the slip-rate exponent n_sl as a compile-time quarter (-1: runtime).
+// MODE 0: dPdF of every cell into 81 planes.  MODE 1 (the solver's
+// C_volAvg and C_minMaxAvg): no dPdF field, only each cell's |dPdF|^2 and
+// per 16-cell row of a block the sums of the 81 planes, in a fixed order.
+// MODE 2:
+// dPdF of up to 64 listed cells (one block), e.g. the two extreme cells
+// that MODE 1 does not keep: the same per-lane arithmetic, so the same values.
+template <int NE4, int MODE>
 __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
   const MatConst& mc = a.mc;
   const MpState& s = a.s;
   const int64_t N = s.N;
-  const int64_t c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.c_end) return;
+  int64_t c;
+  bool live;
+  if constexpr (MODE == 2) {
+    if ((int64_t)threadIdx.x >= a.c_end - a.c_begin) return;
+    c = a.cells[threadIdx.x];
+    live = true;
+  } else {
+    c = a.c_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    live = c < a.c_end;
+    if constexpr (MODE == 0) {
+      if (!live) return;
+    } else {
+      // the whole wave takes part in the block sums: a lane past the end
+      // computes the last cell again and contributes nothing
+      if (!live) c = a.c_end - 1;
+    }
+  }
+  [[maybe_unused]] double n2acc = 0.0;
   const double dt = a.dt;
   double F[9], Fp[9], Fp0[9], S[9], Fe[9], xi[NSLIP];
   load33(s.F, N, c, F);
@@ -1454,6 +1494,7 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) dFt[3 * i + j] = dF[3 * j + i];
       mm33(u3, dFt, a3);
+      [[maybe_unused]] double col[9];  // MODE 1: this column's values, summed over the wave together
 #pragma unroll
       for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -1461,9 +1502,36 @@ __global__ void __launch_bounds__(64) mp_tangent_kernel(TangentArgs a) {
           double v = (k == p) ? d0[3 * l + o] : 0.0;  // dPdF(p,l,p,o) = d0(l,o)
           v = v + a1[3 * k + l] + a2[3 * k + l] + a3[3 * k + l];
           // plane index of dPdF(k,l,p,o): k + 3l + 9p + 27o
-          a.dPdF[(int64_t)(k + 3 * l + 9 * p + 27 * o) * N + c] = v;
+          const int q = k + 3 * l + 9 * p + 27 * o;
+          if constexpr (MODE == 0) {
+            a.dPdF[(int64_t)q * N + c] = v;
+          } else if constexpr (MODE == 2) {
+            a.dPdF[81 * threadIdx.x + q] = v;
+          } else {
+            n2acc += v * v;
+            col[3 * k + l] = live ? v : 0.0;
+          }
         }
+      if constexpr (MODE == 1) {
+        // sums over each row of 16 lanes by DPP exchanges (plain vector
+        // moves): every lane of a row ends with the same bits; the rows'
+        // first lanes store them, four partials per block and plane
+        dpp_step<0xB1>(col);   // quad_perm [1,0,3,2]
+        dpp_step<0x4E>(col);   // quad_perm [2,3,0,1]
+        dpp_step<0x141>(col);  // row_half_mirror
+        dpp_step<0x140>(col);  // row_mirror
+        if ((threadIdx.x & 15) == 0) {
+          const int64_t at = 4 * (int64_t)blockIdx.x + (threadIdx.x >> 4);
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) a.bpart[(int64_t)(k + 3 * l + 9 * p + 27 * o) * a.nblk + at] = col[3 * k + l];
+        }
+      }
     }
+  }
+  if constexpr (MODE == 1) {
+    if (live) a.n2[c] = n2acc;
   }
 }
 
@@ -1624,6 +1692,17 @@ void launch_pav_partial(const double* P, int64_t N, double* partial, hipStream_t
   SCA_HIP(hipGetLastError());
 }
 
+// the example material's slip-rate exponent (n_sl = 20) is compiled in: the
+// slip rates in the response kernel's form (x^(n-1) from one powering chain,
+// c_s without a division; equal to the runtime form to rounding), and no
+// pow() call path, whose register saves spill
+template <int MODE>
+static void launch_tangent_mode(const TangentArgs& a, int64_t blocks, hipStream_t st) {
+  if (a.mc.n_e4 == 80) hipLaunchKernelGGL((mp_tangent_kernel<80, MODE>), dim3((unsigned)blocks), dim3(64), 0, st, a);
+  else hipLaunchKernelGGL((mp_tangent_kernel<-1, MODE>), dim3((unsigned)blocks), dim3(64), 0, st, a);
+  SCA_HIP(hipGetLastError());
+}
+
 void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* dPdF, hipStream_t st,
                        int64_t c_begin, int64_t c_end) {
   if (c_end < 0) c_end = s.N;
@@ -1636,14 +1715,47 @@ void launch_mp_tangent(const MpState& s, const MatConst& mc, double dt, double* 
   a.dPdF = dPdF;
   a.c_begin = c_begin;
   a.c_end = c_end;
-  const int blocks = (int)((c_end - c_begin + 63) / 64);
-  // the example material's slip-rate exponent (n_sl = 20) is compiled in:
-  // the slip rates in the response kernel's form (x^(n-1) from one powering
-  // chain, c_s without a division; equal to the runtime form to rounding),
-  // and no pow() call path, whose register saves spill
-  if (mc.n_e4 == 80) hipLaunchKernelGGL(mp_tangent_kernel<80>, dim3(blocks), dim3(64), 0, st, a);
-  else hipLaunchKernelGGL(mp_tangent_kernel<-1>, dim3(blocks), dim3(64), 0, st, a);
-  SCA_HIP(hipGetLastError());
+  a.n2 = a.bpart = nullptr;
+  a.nblk = 0;
+  a.cells = nullptr;
+  launch_tangent_mode<0>(a, (c_end - c_begin + 63) / 64, st);
+}
+
+int64_t tangent_blocks(int64_t N) { return (N + 63) / 64; }
+int64_t tangent_partials(int64_t N) { return 4 * tangent_blocks(N); }
+
+void launch_mp_tangent_reduce(const MpState& s, const MatConst& mc, double dt, double* n2, double* bpart,
+                              hipStream_t st) {
+  if (s.N < 1) return;
+  TangentArgs a;
+  a.s = s;
+  a.mc = mc;
+  a.dt = dt;
+  a.dPdF = nullptr;
+  a.n2 = n2;
+  a.bpart = bpart;
+  a.nblk = 4 * tangent_blocks(s.N);
+  a.cells = nullptr;
+  a.c_begin = 0;
+  a.c_end = s.N;
+  launch_tangent_mode<1>(a, tangent_blocks(s.N), st);
+}
+
+void launch_mp_tangent_cells(const MpState& s, const MatConst& mc, double dt, const int64_t* d_cells, int n,
+                             double* out, hipStream_t st) {
+  if (n < 1) return;
+  if (n > 64) throw std::runtime_error("at most 64 listed cells per tangent launch");
+  TangentArgs a;
+  a.s = s;
+  a.mc = mc;
+  a.dt = dt;
+  a.dPdF = out;
+  a.n2 = a.bpart = nullptr;
+  a.nblk = 0;
+  a.cells = d_cells;
+  a.c_begin = 0;
+  a.c_end = n;
+  launch_tangent_mode<2>(a, 1, st);
 }
 
 void launch_mp_forward(const MpState& s, hipStream_t st) {

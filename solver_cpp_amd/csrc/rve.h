@@ -41,7 +41,7 @@ enum {
   SM_FSUM = 150,  // 9: forward_field sums
   SM_FDIFF = 160, // 9: forward_field correction
   SM_ITERS = 111, // 2: Newton / state iterations of the last response (profiling)
-  SM_G81 = 200    // 162: gathered dPdF of argmax / argmin
+  SM_G81 = 200    // 162: dPdF of the argmax / argmin cells
 };
 
 // HIP-event timing of kernel classes (sc_kernel_times), off by default
@@ -191,7 +191,7 @@ class Rve {
   double *F_last_inc = nullptr, *F_dot = nullptr;
   Ngmres* ngm = nullptr;
   double* spec = nullptr;
-  double* dPdF = nullptr;
+  double* tpart = nullptr;  // the tangent's plane sums per 16 cells (81 x tangent_partials(N))
   double* sub = nullptr;
   double* partial = nullptr;
   double* small = nullptr;

@@ -66,7 +66,7 @@ int64_t rve_footprint(const sc_rve_desc& d) {
   const int64_t N = (int64_t)d.cells[0] * d.cells[1] * d.cells[2];
   const int64_t K = (int64_t)(d.cells[0] / 2 + 1) * d.cells[1] * d.cells[2];
   const int m = d.grid_num.ngmres_m > 0 ? d.grid_num.ngmres_m : 30;
-  int64_t doubles = 362 * N + 18 * K + (int64_t)18 * m * N + 3 * (int64_t)(d.cells[0] + 1) * (d.cells[1] + 1) *
+  int64_t doubles = 288 * N + 18 * K + (int64_t)18 * m * N + 3 * (int64_t)(d.cells[0] + 1) * (d.cells[1] + 1) *
                                                                (d.cells[2] + 1);
   if (d.grid_num.derivative != 0) doubles += 6 * K;
   if (d.precision == SC_PRECISION_MIXED) doubles += 9 * K;

@@ -371,7 +371,7 @@ void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
   F_last_inc = dalloc(9 * N);
   F_dot = dalloc(9 * N);
   spec = dalloc(2 * 9 * K);
-  dPdF = dalloc(81 * N);
+  tpart = dalloc((size_t)81 * tangent_partials(N));
   partial = dalloc((size_t)std::max<int64_t>(red_blocks() * MAX_RED_VEC, 9 * mp_partial_blocks(N)) +
                    gamma_blocks(K));
   if (residual_graphs && N <= RESIDUAL_GRAPH_MAX_CELLS) {  // staging of the captured residual evaluation
@@ -574,36 +574,54 @@ void Rve::fetch_response_scalars(bool rotate) {
 }
 
 // consistent tangent of the last evaluation: C_volAvg, C_minMaxAvg
-// (spectral.cpp:189-217)
+// (spectral.cpp:189-217).  The tangent is reduced where it is formed (no
+// dPdF field): per-cell norms and per-block plane sums; the two cells of
+// the extreme norms are computed again for their dPdF.
 void Rve::tangent() {
   complete_response_scalars();
   sync_response_F();
   kt.begin(SC_KT_TANGENT, st);
-  launch_mp_tangent(mp, mc, last_dt, dPdF, st);
+  launch_mp_tangent_reduce(mp, mc, last_dt, d_n2, tpart, st);
   kt.end(SC_KT_TANGENT, N, st);
-  // one pass over dPdF: the cells of the extreme norms and the plane sums
-  int64_t imx = -1, imn = -1;
+  // C_volAvg: the blocks' plane sums in the fixed reduction trees
   const int rb = red_blocks();
-  tangent_minmax_select(dPdF, N, d_n2, partial + (size_t)81 * rb, d_minmax_i, imx, imn, st, partial);
-  double sums[81];
+  const int64_t nb = tangent_partials(N);
+  int ops[81];
+  const double* av[81];
+  for (int q = 0; q < 81; ++q) {
+    ops[q] = 2;
+    av[q] = tpart + (size_t)q * nb;
+  }
+  launch_partial_reduce<double>(ops, nb, 81, av, nullptr, partial, st);
   launch_final_reduce(partial, 81, rb, small, st);
+  double sums[81];
   SCA_HIP(hipMemcpyAsync(sums, small, sizeof sums, hipMemcpyDeviceToHost, st));
-  SCA_HIP(hipStreamSynchronize(st));
+  // the cells of the extreme norms (the selection synchronises the stream)
+  int64_t imx = -1, imn = -1;
+  tangent_select_norms(d_n2, N, partial + (size_t)81 * rb, d_minmax_i, imx, imn, st);
   for (int q = 0; q < 81; ++q) C_volAvg[q] = sums[q] * wgt;
   double dmax[81], dmin[81];
   for (int q = 0; q < 81; ++q) {
     dmax[q] = 0.0;
     dmin[q] = std::numeric_limits<double>::max();
   }
-  if (imx >= 0) {
-    launch_gather81(dPdF, N, imx, small + SM_G81, st);
-    SCA_HIP(hipMemcpyAsync(dmax, small + SM_G81, 81 * sizeof(double), hipMemcpyDeviceToHost, st));
+  int64_t cl[2];
+  int n = 0;
+  if (imx >= 0) cl[n++] = imx;
+  if (imn >= 0) cl[n++] = imn;
+  for (int i = 0; i < n; ++i)
+    if (cl[i] >= N) throw std::runtime_error("tangent: selected cell out of range");
+  if (n > 0) {
+    static_assert(sizeof(long long) == sizeof(int64_t), "cell list in the index scratch");
+    SCA_HIP(hipMemcpyAsync(d_minmax_i, cl, n * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    launch_mp_tangent_cells(mp, mc, last_dt, reinterpret_cast<const int64_t*>(d_minmax_i), n, small + SM_G81, st);
+    double h[162];
+    SCA_HIP(hipMemcpyAsync(h, small + SM_G81, n * 81 * sizeof(double), hipMemcpyDeviceToHost, st));
+    SCA_HIP(hipStreamSynchronize(st));
+    int i = 0;
+    if (imx >= 0) std::memcpy(dmax, h + 81 * i++, sizeof dmax);
+    if (imn >= 0) std::memcpy(dmin, h + 81 * i, sizeof dmin);
   }
-  if (imn >= 0) {
-    launch_gather81(dPdF, N, imn, small + SM_G81 + 81, st);
-    SCA_HIP(hipMemcpyAsync(dmin, small + SM_G81 + 81, 81 * sizeof(double), hipMemcpyDeviceToHost, st));
-  }
-  SCA_HIP(hipStreamSynchronize(st));
   for (int q = 0; q < 81; ++q) C_minMaxAvg[q] = 0.5 * (dmax[q] + dmin[q]);
   kt.flush();
 }

@@ -81,13 +81,12 @@ void launch_shared_reduce(int mode, int64_t n, const T* u, const T* w, int nv, c
 // cells of C_minMaxAvg: first cell of largest / smallest |dPdF|^2 with norms
 // within TANGENT_TIE_REL of the extreme counted as tied.  n2: N doubles of
 // scratch, pv: 2*red_blocks() doubles, pi: 2*red_blocks() long longs.
-// sums (optional, apart from pv): the same pass also leaves the 81 planes'
-// partial sums there (81 x red_blocks(), as launch_partial_reduce op 2).
 constexpr double TANGENT_TIE_REL = 1e-10;
 void tangent_minmax_select(const double* dPdF, int64_t N, double* n2, double* pv, long long* pi, int64_t& imax,
-                           int64_t& imin, hipStream_t st,
-                           double* sums = nullptr);
-void launch_gather81(const double* d, int64_t N, int64_t c, double* out, hipStream_t st);
+                           int64_t& imin, hipStream_t st);
+// the same selection from per-cell norms n2 already formed (pv, pi as above)
+void tangent_select_norms(const double* n2, int64_t N, double* pv, long long* pi, int64_t& imax, int64_t& imin,
+                          hipStream_t st);
 void launch_bc_aim(const double* sums, const double S[81], const double R[9], double wgt, int rotate, double* aim,
                    hipStream_t st);
 void launch_final_reduce(const double* partial, int nvec, int nb, double* out, hipStream_t st);

@@ -1397,67 +1397,6 @@ __global__ void __launch_bounds__(256) dpdf_norm_partial(const double* d, int64_
     pv[2 * blockIdx.x + 1] = vmn[0];
   }
 }
-// dpdf_norm_partial fused with the sums of the 81 planes (C_volAvg): dPdF is
-// read once.  The per-thread element order and the block trees are those of
-// dpdf_norm_partial and of partial_reduce_kernel (op 2), so n2, the block
-// extremes and sums[q * RED_BLOCKS + b] are bitwise theirs.
-__global__ void __launch_bounds__(256) dpdf_reduce_partial(const double* d, int64_t N, double* n2, double* pv,
-                                                           double* sums) {
-  __shared__ double red[9][256];
-  double acc[81];
-#pragma unroll
-  for (int q = 0; q < 81; ++q) acc[q] = 0.0;
-  double bmx = -1.0, bmn = 1.7976931348623157e308;
-  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < N; c += (int64_t)RED_BLOCKS * 256) {
-    double s = 0.0;
-#pragma unroll
-    for (int q0 = 0; q0 < 81; q0 += 9) {
-      double x[9];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) x[j] = d[(q0 + j) * N + c];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) {
-        s += x[j] * x[j];
-        acc[q0 + j] += x[j];
-      }
-    }
-    n2[c] = s;
-    bmx = fmax(bmx, s);
-    bmn = fmin(bmn, s);
-  }
-  const int t = threadIdx.x;
-  red[0][t] = bmx;
-  red[1][t] = bmn;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) {
-      red[0][t] = fmax(red[0][t], red[0][t + w]);
-      red[1][t] = fmin(red[1][t], red[1][t + w]);
-    }
-    __syncthreads();
-  }
-  if (t == 0) {
-    pv[2 * blockIdx.x] = red[0][0];
-    pv[2 * blockIdx.x + 1] = red[1][0];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q0 = 0; q0 < 81; q0 += 9) {
-#pragma unroll
-    for (int j = 0; j < 9; ++j) red[j][t] = acc[q0 + j];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if (t < w) {
-#pragma unroll
-        for (int j = 0; j < 9; ++j) red[j][t] += red[j][t + w];
-      }
-      __syncthreads();
-    }
-    if (t < 9) sums[(int64_t)(q0 + t) * RED_BLOCKS + blockIdx.x] = red[t][0];
-    __syncthreads();
-  }
-}
-
 // Pass 2: per block the first cell with n2 >= hi and the first with n2 <= lo (-1: none)
 __global__ void __launch_bounds__(256) first_index_partial(const double* n2, int64_t N, double hi, double lo,
                                                            long long* pi) {
@@ -1485,14 +1424,54 @@ __global__ void __launch_bounds__(256) first_index_partial(const double* n2, int
     pi[2 * blockIdx.x + 1] = imn[0];
   }
 }
+// block extremes of given per-cell norms (the tree of dpdf_norm_partial)
+__global__ void __launch_bounds__(256) norm_extremes_partial(const double* n2, int64_t N, double* pv) {
+  __shared__ double vmx[256], vmn[256];
+  double bmx = -1.0, bmn = 1.7976931348623157e308;
+  for (int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x; c < N; c += (int64_t)RED_BLOCKS * 256) {
+    const double s = n2[c];
+    bmx = fmax(bmx, s);
+    bmn = fmin(bmn, s);
+  }
+  const int t = threadIdx.x;
+  vmx[t] = bmx;
+  vmn[t] = bmn;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      vmx[t] = fmax(vmx[t], vmx[t + w]);
+      vmn[t] = fmin(vmn[t], vmn[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    pv[2 * blockIdx.x] = vmx[0];
+    pv[2 * blockIdx.x + 1] = vmn[0];
+  }
+}
+
+static void select_from_extremes(const double* n2, int64_t N, double* pv, long long* pi, int64_t& imax,
+                                 int64_t& imin, hipStream_t st);
+
 void tangent_minmax_select(const double* dPdF, int64_t N, double* n2, double* pv, long long* pi, int64_t& imax,
-                           int64_t& imin, hipStream_t st, double* sums) {
-  const int nb = RED_BLOCKS;
-  if (sums)
-    hipLaunchKernelGGL(dpdf_reduce_partial, dim3(nb), dim3(256), 0, st, dPdF, N, n2, pv, sums);
-  else
-    hipLaunchKernelGGL(dpdf_norm_partial, dim3(nb), dim3(256), 0, st, dPdF, N, n2, pv);
+                           int64_t& imin, hipStream_t st) {
+  hipLaunchKernelGGL(dpdf_norm_partial, dim3(RED_BLOCKS), dim3(256), 0, st, dPdF, N, n2, pv);
   SCA_HIP(hipGetLastError());
+  select_from_extremes(n2, N, pv, pi, imax, imin, st);
+}
+
+void tangent_select_norms(const double* n2, int64_t N, double* pv, long long* pi, int64_t& imax, int64_t& imin,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(norm_extremes_partial, dim3(RED_BLOCKS), dim3(256), 0, st, n2, N, pv);
+  SCA_HIP(hipGetLastError());
+  select_from_extremes(n2, N, pv, pi, imax, imin, st);
+}
+
+// the global extremes from the block extremes in pv, then the first cell
+// within TANGENT_TIE_REL of each
+static void select_from_extremes(const double* n2, int64_t N, double* pv, long long* pi, int64_t& imax,
+                                 int64_t& imin, hipStream_t st) {
+  const int nb = RED_BLOCKS;
   std::vector<double> h(2 * nb);
   SCA_HIP(hipMemcpyAsync(h.data(), pv, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, st));
   SCA_HIP(hipStreamSynchronize(st));
@@ -1513,15 +1492,6 @@ void tangent_minmax_select(const double* dPdF, int64_t N, double* n2, double* pv
     if (x >= 0 && (imax < 0 || x < imax)) imax = x;
     if (y >= 0 && (imin < 0 || y < imin)) imin = y;
   }
-}
-// gather the 81 planes of one point
-__global__ void gather81_kernel(const double* d, int64_t N, int64_t c, double* out) {
-  const int q = threadIdx.x;
-  if (q < 81) out[q] = d[q * N + c];
-}
-void launch_gather81(const double* d, int64_t N, int64_t c, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(gather81_kernel, dim3(1), dim3(128), 0, st, d, N, c, out);
-  SCA_HIP(hipGetLastError());
 }
 // dF_aim on the device: P_av = wgt * sums (rotated R P R^T when rotating),
 // dF = S : P_av in the load frame, aim = R^T dF R in the field frame (active)
