@@ -88,11 +88,14 @@ class Ngmres {
   // called after every synchronisation of the reductions (the owner takes up
   // results it copied asynchronously)
   std::function<void()> after_sync;
+  // waits for the stream (the owner's choice of waiting); reductions land
+  // in h_out, pinned host memory the final reduction kernel writes
+  std::function<void()> wait;
   // the point a solve returned was the point of its last evaluation
   bool solution_evaluated_last = false;
   int64_t n;
   hipStream_t st;
-  double *partial = nullptr, *small = nullptr;
+  double *partial = nullptr, *h_out = nullptr;
   KTimer* kt = nullptr;
 
  private:
@@ -172,6 +175,10 @@ class Rve {
   int nr;
   double wgt;
   hipStream_t st = nullptr;
+  // polling waits (wait_stream); the job server's contexts wait blocking
+  bool spin_wait = true;
+  hipEvent_t wait_ev = nullptr;
+  void wait_stream();
   MatConst mc;
   MpState mp;
   FftPlan<double> fft;

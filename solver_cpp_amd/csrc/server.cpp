@@ -283,6 +283,7 @@ class Server {
           // jobs evaluate eagerly: the per-job streams keep the device busy,
           // and short jobs would not amortise graph captures
           j->rve.reset(new Rve(j->desc, false));
+          j->rve->spin_wait = false;  // the workers share the host cores
           j->rve->batch = batcher_.get();
           if (j->evicted) {
             j->rve->load(j->snapshot);

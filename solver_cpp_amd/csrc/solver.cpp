@@ -339,8 +339,10 @@ void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
   st = stream_get();
   fft.init(cells);
   if (d.grid_num.fft_plan_mode == 1) fft.tune(9, st);
+  SCA_HIP(hipEventCreateWithFlags(&wait_ev, hipEventDisableTiming));
   ngm = new Ngmres(9 * N, st, &kt);
   ngm->after_sync = [this] { complete_response_scalars(); };
+  ngm->wait = [this] { wait_stream(); };
   h_scal = static_cast<double*>(host_alloc(16 * sizeof(double)));
   mixed = d.precision == SC_PRECISION_MIXED;
   if (mixed) {
@@ -463,6 +465,8 @@ void Rve::release() {
   release_residual_graph();
   delete ngm;
   ngm = nullptr;
+  if (wait_ev) (void)hipEventDestroy(wait_ev);
+  wait_ev = nullptr;
   host_free(h_scal);
   h_scal = nullptr;
   dev_free(spec32);
@@ -540,9 +544,30 @@ void Rve::request_response_scalars(bool rotate, bool residual) {
   scal_reused = false;
 }
 
+// Waits for the work issued on the stream.  A solve waits about twice per
+// N-GMRES iteration, each time for a few scalars: a standalone context polls
+// an event (bounded, then the blocking wait), which wakes up sooner than the
+// runtime's yielding wait; jobs of the server, whose worker threads share the
+// host cores, wait blocking.
+void Rve::wait_stream() {
+  if (!spin_wait) {
+    SCA_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  SCA_HIP(hipEventRecord(wait_ev, st));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (;;) {
+    const hipError_t e = hipEventQuery(wait_ev);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) SCA_HIP(e);
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
+  }
+  SCA_HIP(hipEventSynchronize(wait_ev));
+}
+
 void Rve::complete_response_scalars() {
   if (!scal_pending) return;
-  SCA_HIP(hipStreamSynchronize(st));
+  wait_stream();
   scal_pending = false;
   int ill = 0;
   std::memcpy(&ill, &h_scal[9], sizeof ill);
@@ -850,11 +875,13 @@ Ngmres::Ngmres(int64_t n_, hipStream_t st_, KTimer* kt_) : n(n_), st(st_), kt(kt
     bufs.push_back(p);
   };
   a(partial, (size_t)red_blocks() * MAX_RED_VEC);
-  a(small, MAX_RED_VEC);
+  h_out = static_cast<double*>(host_alloc(MAX_RED_VEC * sizeof(double)));
+  wait = [this] { SCA_HIP(hipStreamSynchronize(st)); };
 }
 Ngmres::~Ngmres() {
   for (double* p : bufs) dev_free(p);  // the owner synchronised the stream
   for (double* p : pool) dev_free(p);
+  host_free(h_out);
 }
 double* Ngmres::take() {
   auto used = [&](const double* p) {
@@ -881,10 +908,10 @@ std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vecto
     if (kt) kt->begin(SC_KT_REDUCE, st);
     launch_partial_reduce<T>(ops.data() + v0, n, nv, a.data() + v0, b.empty() ? nullptr : b.data() + v0, partial,
                              st);
-    launch_final_reduce(partial, nv, red_blocks(), small, st);
+    launch_final_reduce(partial, nv, red_blocks(), h_out, st);  // into pinned host memory
     if (kt) kt->end(SC_KT_REDUCE, n * nv, st);
-    SCA_HIP(hipMemcpyAsync(out.data() + v0, small, nv * sizeof(double), hipMemcpyDeviceToHost, st));
-    SCA_HIP(hipStreamSynchronize(st));
+    wait();
+    std::memcpy(out.data() + v0, h_out, nv * sizeof(double));
   }
   if (after_sync) after_sync();
   if (kt) kt->flush();
@@ -893,10 +920,10 @@ std::vector<double> Ngmres::reduce(const std::vector<int>& ops, const std::vecto
 
 std::vector<double> Ngmres::collect(int nout, int64_t units) {
   std::vector<double> out(nout);
-  launch_final_reduce(partial, nout, red_blocks(), small, st);
+  launch_final_reduce(partial, nout, red_blocks(), h_out, st);  // into pinned host memory
   if (kt) kt->end(SC_KT_REDUCE, units, st);
-  SCA_HIP(hipMemcpyAsync(out.data(), small, nout * sizeof(double), hipMemcpyDeviceToHost, st));
-  SCA_HIP(hipStreamSynchronize(st));
+  wait();
+  std::memcpy(out.data(), h_out, nout * sizeof(double));
   if (after_sync) after_sync();
   if (kt) kt->flush();
   return out;

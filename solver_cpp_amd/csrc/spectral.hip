@@ -692,7 +692,10 @@ __global__ void __launch_bounds__(256) final_reduce_kernel(const double* partial
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[v] = red[0];
+  if (threadIdx.x == 0) {
+    out[v] = red[0];
+    __threadfence_system();  // out may be pinned host memory, read once the stream is done
+  }
 }
 
 // final reductions of a batch: blockIdx.y is the entry, blockIdx.x the
