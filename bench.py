@@ -48,17 +48,19 @@ FP64_PEAK_TFLOPS = 78.6
 
 # Algorithmic work of the material-point update (own count of this
 # formulation; the kernel counts its Newton and state iterations): FP64
-# flops per Newton iteration (Fe, Hooke, 12 slip rates, residual, traceless
-# 8x8 Jacobian assembly 1610, LU 470), per state iteration and per cell, and
-# bytes of state read and written per cell and evaluation (F F0 Fp0 Lp S0
-# xi0 gamma0 in, Lp S P Fp Fe xi gamma out, 2 counters).  The FP64
-# instructions rocprofv3 counts over the timed launches of this workload
-# (x 64 lanes x the active-lane share) give fewer flops than this count
-# (the compiler folds the slip systems' 0 / +-1 coefficients); the ratio is
-# recorded in profiles/ and scales the count.
-RESPONSE_FLOPS_NEWTON = 2800
-RESPONSE_FLOPS_STATE = 850
-RESPONSE_FLOPS_CELL = 300
+# flops per Newton iteration (Fe, Hooke, 12 resolved shears and slip rates,
+# residual, the 8x8 Jacobian from the Hadamard sums of the slip groups and
+# its LU, as the compiled loop body executes them), per state iteration
+# (the rest of integrateStress, dotState with its quarter powers, the damped
+# update) and per cell (entry, restore, first rates, stores), and bytes of
+# state read and written per cell and evaluation (F F0 Fp0 Lp S0 xi0 gamma0
+# in, Lp S P Fp Fe xi gamma out, 2 counters).  The FP64 instructions
+# rocprofv3 counts over the timed launches of this workload (x 64 lanes x
+# the active-lane share) differ from this count (line-search iterations
+# skip the Jacobian); the ratio is recorded in profiles/ and scales the count.
+RESPONSE_FLOPS_NEWTON = 1500
+RESPONSE_FLOPS_STATE = 1000
+RESPONSE_FLOPS_CELL = 500
 RESPONSE_BYTES_PER_CELL = (9 * 5 + 12 * 2 + 9 * 5 + 12 * 2) * 8 + 8
 
 # HBM bytes per launch of the response kernel on this workload, measured by
