@@ -339,6 +339,7 @@ int sc_rve_result_open(sc_rve* R, const char* path, const sc_result_desc* d) {
       R->r->sync_response_F();
       std::vector<double> u_p, u_n;
       rve_displacements(*R->r, u_p, u_n);
+      R->r->refresh_state();
       R->result->increment(inc, t, R->r->mp, R->r->st, u_p, u_n);
     };
     return 0;
@@ -451,6 +452,7 @@ int sc_rve_get_field(sc_rve* R, int field, double* out, int64_t n) {
 
 // host copy of a field in the reference's layout (sc_rve_get_field)
 int sca::rve_get_field(sca::Rve& v, int field, double* out, int64_t n) {
+  v.refresh_state();
   {
     const int64_t N = v.N;
     const double* src = nullptr;

@@ -160,6 +160,8 @@ void launch_mp_tangent_reduce(const MpState& s, const MatConst& mc, double dt, d
 void launch_mp_tangent_cells(const MpState& s, const MatConst& mc, double dt, const int64_t* d_cells, int n,
                              double* out, hipStream_t st);
 void launch_mp_forward(const MpState& s, hipStream_t st);
+// what = 1: F0 <- F, Lp0 <- Lp; what = 2: Fp, S, xi, gamma <- their start values
+void launch_mp_copy(const MpState& s, int what, hipStream_t st);
 void launch_mp_init(const MpState& s, const double* d_Fp_init /*(9,N)*/, double xi0, hipStream_t st);
 
 }  // namespace sca

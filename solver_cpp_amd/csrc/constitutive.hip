@@ -1554,6 +1554,34 @@ __global__ void mp_forward_kernel(MpState s) {
   }
 }
 
+// the pieces of f_materialpoint_forward a context with swappable buffers
+// needs: what = 1: F0 <- F, Lp0 <- Lp (the current values stay in use: the
+// iterate and the Newton start); what = 2: Fp <- Fp0, S <- S0, xi <- xi0,
+// gamma <- gamma0 (the current values again equal to the start values)
+__global__ void mp_copy_kernel(MpState s, int what) {
+  const int64_t N = s.N;
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  if (what == 1) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      s.F0[q * N + c] = s.F[q * N + c];
+      s.Lp0[q * N + c] = s.Lp[q * N + c];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      s.Fp[q * N + c] = s.Fp0[q * N + c];
+      s.S[q * N + c] = s.S0[q * N + c];
+    }
+#pragma unroll
+    for (int k = 0; k < NSLIP; ++k) {
+      s.xi[k * N + c] = s.xi0[k * N + c];
+      s.gamma[k * N + c] = s.gamma0[k * N + c];
+    }
+  }
+}
+
 // mechanical_init (phase_mechanical.f90:253-262) + phenopowerlaw state init
 __global__ void mp_init_kernel(MpState s, const double* Fp_init, double xi_0) {
   const int64_t N = s.N;
@@ -1760,6 +1788,12 @@ void launch_mp_tangent_cells(const MpState& s, const MatConst& mc, double dt, co
 
 void launch_mp_forward(const MpState& s, hipStream_t st) {
   hipLaunchKernelGGL(mp_forward_kernel, dim3((unsigned)((s.N + 255) / 256)), dim3(256), 0, st, s);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_mp_copy(const MpState& s, int what, hipStream_t st) {
+  if (what != 1 && what != 2) throw std::runtime_error("material-point copy: what");
+  hipLaunchKernelGGL(mp_copy_kernel, dim3((unsigned)((s.N + 255) / 256)), dim3(256), 0, st, s, what);
   SCA_HIP(hipGetLastError());
 }
 

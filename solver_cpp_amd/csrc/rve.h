@@ -175,6 +175,10 @@ class Rve {
   int nr;
   double wgt;
   hipStream_t st = nullptr;
+  // after a forward step that exchanged the roles of the current and start
+  // buffers of Fp, S, xi, gamma, until the next material-point update
+  bool state_stale = false;
+  void refresh_state();  // makes the current set readable (a copy, if stale)
   // polling waits (wait_stream); the job server's contexts wait blocking
   bool spin_wait = true;
   hipEvent_t wait_ev = nullptr;

@@ -503,7 +503,16 @@ std::vector<double> Rve::reduce(const std::vector<int>& ops, const std::vector<c
 
 // Spectral::constitutive_response (spectral.cpp:132-219) up to the P_av sums
 // on the device; the tangent reductions are deferred to tangent().
+// the current Fp, S, xi and gamma after a forward step that exchanged the
+// buffers' roles: the start values (what the forward copy would have left)
+void Rve::refresh_state() {
+  if (!state_stale) return;
+  launch_mp_copy(mp, 2, st);
+  state_stale = false;
+}
+
 void Rve::constitutive_response(const double* Fdev, double dt) {
+  state_stale = false;  // the update writes the current set of every cell
   MpState s = mp;
   s.F = const_cast<double*>(Fdev);  // read only
   f_eval = Fdev;
@@ -676,6 +685,7 @@ GammaHost Rve::gamma_host(bool do_gamma) const {
 // scalars.  Its launches depend on F, r, delta_t, the compliance S, the load
 // rotation, C_ref and the precision phase only, so it can be captured.
 void Rve::launch_residual(const double* Fdev, void* rdev) {
+  state_stale = false;  // the update writes the current set of every cell
   MpState s = mp;
   s.F = const_cast<double*>(Fdev);  // read only
   kt.begin(SC_KT_RESPONSE, st);
@@ -818,6 +828,7 @@ void Rve::form_residual(const double* Fdev, void* rdev) {
     launch_spectral(rdev);
   } else if (batch) {
     SCA_HIP(hipStreamSynchronize(st));  // F and everything before it complete
+    state_stale = false;                 // the joint update writes the current set
     batch->evaluate(this, Fdev, rdev);   // the response scalars land in h_scal
   } else if (g_in && !kt.on && ensure_residual_graph()) {
     SCA_HIP(hipMemcpyAsync(g_in, Fdev, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -1225,6 +1236,7 @@ struct CkptHeader {
 void Rve::save(std::vector<char>& blob) {
   complete_response_scalars();
   sync_response_F();
+  refresh_state();
   const std::vector<std::pair<double*, int64_t>> planes = {
       {mp.F, 9 * N},       {mp.F0, 9 * N},    {mp.Fp0, 9 * N},   {mp.Lp0, 9 * N},   {mp.S0, 9 * N},
       {mp.xi0, 12 * N},    {mp.gamma0, 12 * N}, {mp.Lp, 9 * N},  {mp.P, 9 * N},     {mp.Fp, 9 * N},
@@ -1297,6 +1309,7 @@ void Rve::load(const std::vector<char>& blob) {
   complete_response_scalars();
   release_residual_graph();
   g_seen.clear();
+  state_stale = false;  // both sets of the state are restored
   const std::vector<std::pair<double*, int64_t>> planes = {
       {mp.F, 9 * N},       {mp.F0, 9 * N},    {mp.Fp0, 9 * N},   {mp.Lp0, 9 * N},   {mp.S0, 9 * N},
       {mp.xi0, 12 * N},    {mp.gamma0, 12 * N}, {mp.Lp, 9 * N},  {mp.P, 9 * N},     {mp.Fp, 9 * N},
@@ -1390,7 +1403,22 @@ void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_r
     last_uniform = x_uniform;
     std::memcpy(last_value, x_value, sizeof last_value);
     sync_response_F();
-    launch_mp_forward(mp, st);  // f_materialpoint_forward
+    // f_materialpoint_forward.  Fp, S, xi and gamma become the start values
+    // by exchanging the buffers' roles (the next evaluation writes the
+    // current set before anything reads it; refresh_state() copies it back
+    // for a reader that comes first); F and Lp stay in use and are copied.
+    // A captured residual graph holds the buffer addresses: with it the
+    // whole state is copied.
+    if (g_in) {
+      launch_mp_forward(mp, st);
+    } else {
+      launch_mp_copy(mp, 1, st);
+      std::swap(mp.Fp, mp.Fp0);
+      std::swap(mp.S, mp.S0);
+      std::swap(mp.xi, mp.xi0);
+      std::swap(mp.gamma, mp.gamma0);
+      state_stale = true;
+    }
   }
   for (int q = 0; q < 9; ++q) F_aim[q] = F_aim_last_inc[q] + F_aim_dot[q] * dt;
   for (int q = 0; q < 9; ++q)
