@@ -50,6 +50,7 @@ Batcher::~Batcher() {
     g.f32.release();
     dev_free(g.real);
     dev_free(g.spec);
+    dev_free(g.rowsum);
   }
   dev_free(d_scal_);
   host_free(h_scal_);
@@ -135,11 +136,14 @@ void Batcher::ensure(Group& g, int cap) {
   const int64_t K = (int64_t)(g.cells[0] / 2 + 1) * g.cells[1] * g.cells[2];
   dev_free(g.real);  // the batch that used them has completed
   dev_free(g.spec);
+  dev_free(g.rowsum);
   g.real = nullptr;
   g.spec = nullptr;
+  g.rowsum = nullptr;
   g.cap = 0;
   g.real = static_cast<double*>(dev_alloc((size_t)9 * N * cap * sizeof(double)));
   g.spec = dev_alloc((size_t)2 * 9 * K * cap * (g.fp32 ? sizeof(float) : sizeof(double)));
+  g.rowsum = static_cast<double*>(dev_alloc((size_t)9 * g.cells[1] * g.cells[2] * cap * sizeof(double)));
   g.cap = cap;
 }
 
@@ -147,17 +151,12 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
   const int n = (int)reqs.size();
   if (n == 0) return;
   ta_.reserve((size_t)n * batch_table_bytes_per_rve() + (64 << 10));
-  // material point, P block sums, P sums, aims
-  std::vector<MpBatchItem> mp(n);
-  std::vector<BatchReduce> red(n);
-  std::vector<BcBatchItem> bc(n);
-  for (int i = 0; i < n; ++i) {
-    Rve& v = *reqs[i]->rve;
-    MpState s = v.mp;
-    s.F = const_cast<double*>(reqs[i]->F);  // read only
-    mp[i] = MpBatchItem{s, v.mc, v.delta_t, v.d_ill, v.sub, v.partial};
-    red[i] = BatchReduce{v.partial, mp_partial_blocks(v.N), 9, v.small + SM_SUMS};
-    BcBatchItem& b = bc[i];
+  // material point, P sums, aims.  A context whose x length has a pencil
+  // kernel takes its P sums from the row sums of its forward transform (as
+  // alone, Rve::launch_spectral), so its aim follows that transform; the
+  // others sum P after the update, as alone.
+  auto bc_item = [](Rve& v) {
+    BcBatchItem b;
     b.sums = v.small + SM_SUMS;
     std::memcpy(b.S, v.S, sizeof b.S);
     double R[9];
@@ -167,10 +166,24 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
     b.wgt = v.wgt;
     b.rotate = 1;
     b.aim = v.small + SM_AIM;
+    return b;
+  };
+  std::vector<MpBatchItem> mp(n);
+  std::vector<BatchReduce> red;
+  std::vector<BcBatchItem> bc;
+  for (int i = 0; i < n; ++i) {
+    Rve& v = *reqs[i]->rve;
+    MpState s = v.mp;
+    s.F = const_cast<double*>(reqs[i]->F);  // read only
+    mp[i] = MpBatchItem{s, v.mc, v.delta_t, v.d_ill, v.sub, v.xsums ? nullptr : v.partial};
+    if (!v.xsums) {
+      red.push_back(BatchReduce{v.partial, mp_partial_blocks(v.N), 9, v.small + SM_SUMS});
+      bc.push_back(bc_item(v));
+    }
   }
   launch_mp_response_batch(mp.data(), n, ta_, st_);
-  launch_final_reduce_batch(red.data(), n, ta_, st_);
-  launch_bc_aim_batch(bc.data(), n, ta_, st_);
+  launch_final_reduce_batch(red.data(), (int)red.size(), ta_, st_);
+  launch_bc_aim_batch(bc.data(), (int)bc.size(), ta_, st_);
   // spectral stages per group of equal grid and arithmetic
   std::map<std::tuple<int, int, int, int>, std::vector<int>> by_grid;
   for (int i = 0; i < n; ++i) {
@@ -208,6 +221,8 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
       std::vector<BatchCopy> gather(m), scatter(m);
       std::vector<GammaBatchItem> gam(m);
       std::vector<BatchReduce> div(m);
+      std::vector<RowSumItem> rsum;
+      std::vector<BcBatchItem> bcx;
       const size_t spec_stride = (size_t)2 * 9 * K * (g.fp32 ? sizeof(float) : sizeof(double));
       for (int j = 0; j < m; ++j) {
         const Req& q = *reqs[idx[c0 + j]];
@@ -220,17 +235,26 @@ void Batcher::launch(const std::vector<Req*>& reqs) {
         gam[j] = GammaBatchItem{static_cast<char*>(g.spec) + spec_stride * j, v.gamma_host(true), v.small + SM_AIM,
                                 v.d_xi_tab, v.partial};
         div[j] = BatchReduce{v.partial, gamma_blocks(K), 1, v.small + SM_DIV};
+        if (v.xsums) {
+          const int64_t nrow = (int64_t)g.cells[1] * g.cells[2];
+          rsum.push_back(RowSumItem{g.rowsum + (size_t)9 * nrow * j, nrow, v.small + SM_SUMS});
+          bcx.push_back(bc_item(v));
+        }
       }
+      // the contexts of a group share the grid: all or none take row sums
+      double* rowsum = rsum.empty() ? nullptr : g.rowsum;
+      if (!rsum.empty() && (int)rsum.size() != m) throw std::runtime_error("batch: mixed P-sum paths in one group");
       launch_copy_batch(gather.data(), m, ta_, st_);
       // the x passes pair rows within each RVE's 9 planes (set_comp 9): the
       // transforms give the bits of the RVE transformed alone
-      if (g.fp32) {
-        g.f32.forward(g.real, static_cast<float*>(g.spec), 9 * m, st_, 9);
-        launch_gamma_div_batch<float>(gam.data(), m, K, ta_, st_);
-      } else {
-        g.f64.forward(g.real, static_cast<double*>(g.spec), 9 * m, st_, 9);
-        launch_gamma_div_batch<double>(gam.data(), m, K, ta_, st_);
+      if (g.fp32) g.f32.forward(g.real, static_cast<float*>(g.spec), 9 * m, st_, 9, rowsum);
+      else g.f64.forward(g.real, static_cast<double*>(g.spec), 9 * m, st_, 9, rowsum);
+      if (rowsum) {
+        launch_rowsum_reduce_batch(rsum.data(), m, ta_, st_);
+        launch_bc_aim_batch(bcx.data(), m, ta_, st_);
       }
+      if (g.fp32) launch_gamma_div_batch<float>(gam.data(), m, K, ta_, st_);
+      else launch_gamma_div_batch<double>(gam.data(), m, K, ta_, st_);
       launch_final_reduce_batch(div.data(), m, ta_, st_);
       if (g.fp32) g.f32.backward(static_cast<float*>(g.spec), static_cast<float*>(static_cast<void*>(g.real)), 9 * m,
                                  1.0, st_, 9);

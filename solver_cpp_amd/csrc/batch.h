@@ -52,6 +52,7 @@ class Batcher {
     FftPlan<float> f32;
     double* real = nullptr;  // 9 N doubles per RVE slot (P in; r out, as floats in an FP32 group)
     void* spec = nullptr;    // 9 K complex per RVE slot
+    double* rowsum = nullptr;  // 9 Ny Nz x-pass row sums of P per RVE slot (pencil x lengths)
     int cap = 0;
     int tuned_for = 0;  // components / 9 the launch shapes were measured for
   };

@@ -124,7 +124,7 @@ struct MpBatchItem {
   double dt;
   int* ill;
   double* sub;
-  double* pav_partial;
+  double* pav_partial;  // null: no P block sums for this RVE
 };
 // P block sums of one RVE of a batch (device table entry)
 struct MpBatchPav {

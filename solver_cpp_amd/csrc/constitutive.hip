@@ -1666,10 +1666,11 @@ void launch_mp_response_batch(const MpBatchItem* items, int n, TableArena& ta, h
     const MpBatchItem& it = items[i];
     const MpState& s = it.s;
     if (s.N > ((int64_t)1 << 29)) throw std::runtime_error("material-point planes longer than 2^29 cells");
-    if (!s.repivot || !it.ill || !it.sub || !it.pav_partial) throw std::runtime_error("incomplete batch item");
+    if (!s.repivot || !it.ill || !it.sub) throw std::runtime_error("incomplete batch item");
     const int e = cls(it.mc) == 0 ? k0++ : k1++;
     ka[e] = KernelArgs{s, it.mc, it.dt, it.ill, nullptr, s.repivot, it.sub, 0, s.N};
-    pv[e] = MpBatchPav{s.P, s.N, it.pav_partial, mp_partial_blocks(s.N)};
+    // no pav_partial: the P sums come from elsewhere (no blocks)
+    pv[e] = MpBatchPav{s.P, s.N, it.pav_partial, it.pav_partial ? mp_partial_blocks(s.N) : 0};
     maxN = std::max(maxN, s.N);
   }
   // block offsets: class 0 in off[0..n0], class 1 in off[n0+1..n+1]
@@ -1711,7 +1712,8 @@ void launch_mp_response_batch(const MpBatchItem* items, int n, TableArena& ta, h
                        doff + n0 + 1, n - n0);
     hipLaunchKernelGGL((mp_repivot_batch_kernel<-1, -1>), dim3(x2, n - n0), dim3(64), 0, st, dka + n0);
   }
-  hipLaunchKernelGGL(pav_partial_batch_kernel, dim3((unsigned)pav_blocks), dim3(256), 0, st, dpv, dpvoff, n);
+  if (pav_blocks > 0)
+    hipLaunchKernelGGL(pav_partial_batch_kernel, dim3((unsigned)pav_blocks), dim3(256), 0, st, dpv, dpvoff, n);
   SCA_HIP(hipGetLastError());
 }
 

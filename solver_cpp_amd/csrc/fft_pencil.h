@@ -254,10 +254,15 @@ __device__ __forceinline__ void pencil_rows(int64_t pencil, int64_t set_rows, in
 
 // x pass, real -> half complex, two real rows per complex pencil
 // (z = x_a + i x_b): X_a[k] = (Z[k] + conj Z[N-k]) / 2,
-// X_b[k] = (Z[k] - conj Z[N-k]) / (2 i)
+// X_b[k] = (Z[k] - conj Z[N-k]) / (2 i).  rowsum (optional): the sum of each
+// input row in double precision, in a fixed order (each thread's points in
+// turn, then the pencil's TP lanes by an xor butterfly): the same for every
+// block shape, so that a field's sums can be taken from its forward
+// transform instead of another pass over it.
 template <typename T, typename TI, int N>
 __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, cpx<T>* out, int64_t nrows,
-                                                                 int64_t set_rows, const cpx<T>* tw_g) {
+                                                                 int64_t set_rows, const cpx<T>* tw_g,
+                                                                 double* rowsum) {
   using S = Shape<T, N>;
   constexpr int NR = N / 2 + 1;
   extern __shared__ unsigned char smem_raw[];
@@ -269,10 +274,25 @@ __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, c
   pencil_rows((int64_t)blockIdx.x * P + p, set_rows, nrows, ra, rb);
   cpx<T>* row = lds + p * S::LD;
   cpx<T> v[S::E];
+  double sa = 0.0, sb = 0.0;
 #pragma unroll
   for (int j = 0; j < S::E; ++j) {
     const int e = t + S::TP * j;
-    v[j] = {ra < nrows ? (T)in[ra * N + e] : (T)0, rb < nrows ? (T)in[rb * N + e] : (T)0};
+    const TI xa = ra < nrows ? in[ra * N + e] : (TI)0, xb = rb < nrows ? in[rb * N + e] : (TI)0;
+    v[j] = {(T)xa, (T)xb};
+    sa += (double)xa;
+    sb += (double)xb;
+  }
+  if (rowsum) {
+#pragma unroll
+    for (int off = S::TP / 2; off > 0; off >>= 1) {
+      sa += __shfl_xor(sa, off, S::TP);
+      sb += __shfl_xor(sb, off, S::TP);
+    }
+    if (t == 0) {
+      if (ra < nrows) rowsum[ra] = sa;
+      if (rb < nrows) rowsum[rb] = sb;
+    }
   }
   stages<T, N, S::E, -1, 1>(v, t, row, tw);
   __syncthreads();  // the stages' last reads of the row are done

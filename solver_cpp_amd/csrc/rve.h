@@ -175,6 +175,10 @@ class Rve {
   int nr;
   double wgt;
   hipStream_t st = nullptr;
+  // P sums from the forward transform's x-pass row sums (pencil x lengths)
+  // instead of a pass over P after the material-point update
+  bool xsums = false;
+  double* d_rowsum = nullptr;  // 9 Ny Nz
   // after a forward step that exchanged the roles of the current and start
   // buffers of Fp, S, xi, gamma, until the next material-point update
   bool state_stale = false;

@@ -374,6 +374,8 @@ void Rve::setup(const sc_rve_desc& d, bool residual_graphs) {
   F_dot = dalloc(9 * N);
   spec = dalloc(2 * 9 * K);
   tpart = dalloc((size_t)81 * tangent_partials(N));
+  xsums = fft.row_sums_available();
+  if (xsums) d_rowsum = dalloc((size_t)9 * cells[1] * cells[2]);
   partial = dalloc((size_t)std::max<int64_t>(red_blocks() * MAX_RED_VEC, 9 * mp_partial_blocks(N)) +
                    gamma_blocks(K));
   if (residual_graphs && N <= RESIDUAL_GRAPH_MAX_CELLS) {  // staging of the captured residual evaluation
@@ -691,10 +693,12 @@ void Rve::launch_residual(const double* Fdev, void* rdev) {
   kt.begin(SC_KT_RESPONSE, st);
   launch_mp_response(s, mc, delta_t, d_ill, sub, st);
   kt.end(SC_KT_RESPONSE, N, st);
-  kt.begin(SC_KT_REDUCE, st);
-  launch_pav_partial(mp.P, N, partial, st);
-  launch_final_reduce(partial, 9, mp_partial_blocks(N), small + SM_SUMS, st);
-  kt.end(SC_KT_REDUCE, N, st);
+  if (!xsums) {  // else the forward transform gives the P sums
+    kt.begin(SC_KT_REDUCE, st);
+    launch_pav_partial(mp.P, N, partial, st);
+    launch_final_reduce(partial, 9, mp_partial_blocks(N), small + SM_SUMS, st);
+    kt.end(SC_KT_REDUCE, N, st);
+  }
   if (kt.on) {  // iteration counts of this evaluation
     const int ops[2] = {2, 2};
     const int* it[2] = {mp.iters, mp.iters + N};
@@ -709,16 +713,22 @@ void Rve::launch_residual(const double* Fdev, void* rdev) {
 // with the divergence, inverse transform into r, and the asynchronous copy of
 // the response scalars.
 void Rve::launch_spectral(void* rdev) {
-  // DC of the convolution: rotated S : P_av, computed on the device
+  // DC of the convolution: rotated S : P_av, computed on the device once the
+  // P sums are complete (with xsums, from the forward transform's row sums)
   double R[9], Rrm[9];
   qu2om_host(rot_q, R);
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) Rrm[3 * i + j] = R[i + 3 * j];
-  launch_bc_aim(small + SM_SUMS, S, Rrm, wgt, 1, small + SM_AIM, st);
+  auto sums_and_aim = [&] {
+    if (xsums) launch_rowsum_reduce(RowSumItem{d_rowsum, (int64_t)cells[1] * cells[2], small + SM_SUMS}, st);
+    launch_bc_aim(small + SM_SUMS, S, Rrm, wgt, 1, small + SM_AIM, st);
+  };
+  double* rowsum = xsums ? d_rowsum : nullptr;
   if (use32) {  // FP32 transforms and spectrum from the FP64 P; r out in FP32 (exactly the transform's values)
     kt.begin(SC_KT_FFT_FWD, st);
-    fft32.forward(mp.P, spec32, 9, st);
+    fft32.forward(mp.P, spec32, 9, st, 0, rowsum);
     kt.end(SC_KT_FFT_FWD, N, st);
+    sums_and_aim();
     kt.begin(SC_KT_GAMMA, st);
     launch_gamma_div<float>(spec32, K, gamma_host(true), small + SM_AIM, d_xi_tab, partial, st);
     launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);
@@ -728,8 +738,9 @@ void Rve::launch_spectral(void* rdev) {
     kt.end(SC_KT_FFT_BWD, N, st);
   } else {
     kt.begin(SC_KT_FFT_FWD, st);
-    fft.forward(mp.P, spec, 9, st);
+    fft.forward(mp.P, spec, 9, st, 0, rowsum);
     kt.end(SC_KT_FFT_FWD, N, st);
+    sums_and_aim();
     kt.begin(SC_KT_GAMMA, st);
     launch_gamma_div<double>(spec, K, gamma_host(true), small + SM_AIM, d_xi_tab, partial, st);
     launch_final_reduce(partial, 1, gamma_blocks(K), small + SM_DIV, st);

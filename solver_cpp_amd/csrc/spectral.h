@@ -31,8 +31,13 @@ struct FftPlan {
   // divides the components into fields of that many: the x pass pairs two
   // real rows per complex pencil only within a field, so a field's result
   // does not depend on the fields transformed with it.
+  // rowsum (optional, double input and row_sums_available() only): the sum
+  // of every input row (ncomp x Ny x Nz doubles, row = x line), in an order
+  // that no launch shape changes
   template <typename TI>
-  void forward(const TI* real, T* spec, int ncomp, hipStream_t st, int set_comp = 0) const;
+  void forward(const TI* real, T* spec, int ncomp, hipStream_t st, int set_comp = 0,
+               double* rowsum = nullptr) const;
+  bool row_sums_available() const;  // the x length has a pencil kernel
   template <typename TO>
   void backward(T* spec, TO* real, int ncomp, double scale, hipStream_t st,
                 int set_comp = 0) const;  // spec is clobbered
@@ -41,7 +46,7 @@ struct FftPlan {
  private:
   int shape(int axis) const;
   void x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp, hipStream_t st,
-                  int set_comp) const;
+                  int set_comp, double* rowsum = nullptr) const;
   void strided_pass(cpx<T>* S, int ncomp, int sign, int axis, int shp, hipStream_t st) const;
 };
 
@@ -90,6 +95,14 @@ void tangent_select_norms(const double* n2, int64_t N, double* pv, long long* pi
 void launch_bc_aim(const double* sums, const double S[81], const double R[9], double wgt, int rotate, double* aim,
                    hipStream_t st);
 void launch_final_reduce(const double* partial, int nvec, int nb, double* out, hipStream_t st);
+// the 9 sums of a field from its x-pass row sums: out[q] = sum of the rows
+// q * nrow .. (q + 1) * nrow - 1, in a fixed order (one block per q)
+struct RowSumItem {
+  const double* rowsum;
+  int64_t nrow;  // rows per component (Ny x Nz)
+  double* out;
+};
+void launch_rowsum_reduce(const RowSumItem& item, hipStream_t st);
 
 void launch_fwd_field(const double* last, const double* rate, double dt, double* out, int64_t n, hipStream_t st);
 void launch_sub_plane_const(double* f, const double* diff, int64_t N, hipStream_t st);
@@ -119,6 +132,7 @@ struct BatchReduce {
   double* out;
 };
 void launch_final_reduce_batch(const BatchReduce* items, int n, TableArena& ta, hipStream_t st);
+void launch_rowsum_reduce_batch(const RowSumItem* items, int n, TableArena& ta, hipStream_t st);
 // bitwise copies of `words` 32-bit words per entry (a double is two)
 struct BatchCopy {
   const void* src;

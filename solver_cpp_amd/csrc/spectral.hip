@@ -716,6 +716,29 @@ __global__ void __launch_bounds__(256) final_reduce_batch_kernel(const BatchRedu
   if (threadIdx.x == 0) e.out[v] = red[0];
 }
 
+// component sums from x-pass row sums: block q of an item sums its rows
+// (each thread a stride-256 sequence in order, then the block tree)
+__device__ __forceinline__ void rowsum_block(const RowSumItem& e, int q, double* red) {
+  const double* r = e.rowsum + (int64_t)q * e.nrow;
+  double s = 0.0;
+  for (int64_t t = threadIdx.x; t < e.nrow; t += 256) s += r[t];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) e.out[q] = red[0];
+}
+__global__ void __launch_bounds__(256) rowsum_reduce_kernel(RowSumItem e) {
+  __shared__ double red[256];
+  rowsum_block(e, blockIdx.x, red);
+}
+__global__ void __launch_bounds__(256) rowsum_reduce_batch_kernel(const RowSumItem* __restrict__ tab) {
+  __shared__ double red[256];
+  rowsum_block(tab[blockIdx.y], blockIdx.x, red);
+}
+
 // copies of a batch (gathers and scatters between RVE buffers and a group's
 // contiguous arena): blockIdx.y is the entry
 __global__ void __launch_bounds__(256) copy_batch_kernel(const BatchCopy* __restrict__ tab) {
@@ -961,14 +984,14 @@ static std::vector<int> pencil_block_sizes(int n) {
 // rows: set_rows rows per independently paired set (rows is a multiple)
 template <typename T, typename TI>
 static bool pencil_x_r2c(int n, const TI* in, cpx<T>* out, int64_t rows, int64_t set_rows, const cpx<T>* tw,
-                         int bs, hipStream_t st) {
+                         int bs, hipStream_t st, double* rowsum) {
   switch (n) {
 #define SCA_CASE(N)                                                                                          \
   case N: {                                                                                                  \
     using S = pencil::Shape<T, N>;                                                                              \
     const int64_t pencils = rows / set_rows * ((set_rows + 1) / 2), P = bs / S::TP;                          \
     hipLaunchKernelGGL((pencil::x_r2c_kernel<T, TI, N>), dim3((unsigned)((pencils + P - 1) / P)), dim3(bs),   \
-                       (pencil_lds<T, N>(bs)), st, in, out, rows, set_rows, tw);                             \
+                       (pencil_lds<T, N>(bs)), st, in, out, rows, set_rows, tw, rowsum);                     \
     return true;                                                                                             \
   }
     SCA_PENCIL_LENGTHS(SCA_CASE)
@@ -1071,17 +1094,30 @@ void FftPlan<T>::release() {
 }
 
 template <typename T>
+bool FftPlan<T>::row_sums_available() const {
+  switch (cells[0]) {
+#define SCA_CASE(N) case N:
+    SCA_PENCIL_LENGTHS(SCA_CASE)
+#undef SCA_CASE
+    return true;
+    default: return false;
+  }
+}
+
+template <typename T>
 void FftPlan<T>::x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, int ncomp, int shp, hipStream_t st,
-                            int set_comp) const {
+                            int set_comp, double* rowsum) const {
   const int Nx = cells[0], Ny = cells[1], Nz = cells[2];
   const int64_t rows = (int64_t)ncomp * Ny * Nz;
   const int64_t set_rows = (int64_t)(set_comp > 0 ? set_comp : ncomp) * Ny * Nz;
   if (rows % set_rows != 0) throw std::runtime_error("FFT: components not a multiple of the field size");
   const cpx<T>* t0 = reinterpret_cast<const cpx<T>*>(tw[0]);
+  if (rowsum && (!real_is_double || !row_sums_available()))
+    throw std::runtime_error("FFT: row sums need the pencil x pass over double input");
   bool done;
   if (real_is_double)
-    done = pencil_x_r2c<T, double>(Nx, static_cast<const double*>(real), S, rows, set_rows, t0, shp, st);
-  else done = pencil_x_r2c<T, T>(Nx, static_cast<const T*>(real), S, rows, set_rows, t0, shp, st);
+    done = pencil_x_r2c<T, double>(Nx, static_cast<const double*>(real), S, rows, set_rows, t0, shp, st, rowsum);
+  else done = pencil_x_r2c<T, T>(Nx, static_cast<const T*>(real), S, rows, set_rows, t0, shp, st, nullptr);
   if (!done) {
     FftPassDesc d = make_desc(Nx, -1);
     const size_t sh = 2 * (size_t)shp * Nx * sizeof(cpx<T>);
@@ -1097,9 +1133,9 @@ void FftPlan<T>::x_pass_fwd(const void* real, bool real_is_double, cpx<T>* S, in
 
 template <typename T>
 template <typename TI>
-void FftPlan<T>::forward(const TI* real, T* spec, int ncomp, hipStream_t st, int set_comp) const {
+void FftPlan<T>::forward(const TI* real, T* spec, int ncomp, hipStream_t st, int set_comp, double* rowsum) const {
   cpx<T>* S = reinterpret_cast<cpx<T>*>(spec);
-  x_pass_fwd(real, sizeof(TI) == sizeof(double), S, ncomp, shape(0), st, set_comp);
+  x_pass_fwd(real, sizeof(TI) == sizeof(double), S, ncomp, shape(0), st, set_comp, rowsum);
   strided_passes(S, ncomp, -1, st);
 }
 
@@ -1238,11 +1274,11 @@ void FftPlan<T>::tune(int ncomp, hipStream_t st) {
 
 template struct FftPlan<double>;
 template struct FftPlan<float>;
-template void FftPlan<double>::forward<double>(const double*, double*, int, hipStream_t, int) const;
+template void FftPlan<double>::forward<double>(const double*, double*, int, hipStream_t, int, double*) const;
 template void FftPlan<double>::backward<double>(double*, double*, int, double, hipStream_t, int) const;
-template void FftPlan<float>::forward<double>(const double*, float*, int, hipStream_t, int) const;
+template void FftPlan<float>::forward<double>(const double*, float*, int, hipStream_t, int, double*) const;
 template void FftPlan<float>::backward<double>(float*, double*, int, double, hipStream_t, int) const;
-template void FftPlan<float>::forward<float>(const float*, float*, int, hipStream_t, int) const;
+template void FftPlan<float>::forward<float>(const float*, float*, int, hipStream_t, int, double*) const;
 template void FftPlan<float>::backward<float>(float*, float*, int, double, hipStream_t, int) const;
 
 static GammaArgs gamma_args(const GammaHost& gh, const double* d_xi_tab, const double* d_aim) {
@@ -1576,9 +1612,9 @@ void TableArena::reserve(size_t bytes) {
 
 size_t batch_table_bytes_per_rve() {
   // material point (KernelArgs and P block sums, constitutive.hip) + aim +
-  // Gamma + three copies + three reductions, with alignment slack
+  // Gamma + three copies + three reductions + row sums, with alignment slack
   return 1024 + sizeof(BcBatchEntry) + sizeof(GammaBatchEntry) + 3 * sizeof(BatchCopy) + 3 * sizeof(BatchReduce) +
-         8 * 256;
+         sizeof(RowSumItem) + 9 * 256;
 }
 
 void launch_final_reduce_batch(const BatchReduce* items, int n, TableArena& ta, hipStream_t st) {
@@ -1645,6 +1681,21 @@ void launch_gamma_div_batch(const GammaBatchItem* items, int n, int64_t K, Table
 }
 template void launch_gamma_div_batch<double>(const GammaBatchItem*, int, int64_t, TableArena&, hipStream_t);
 template void launch_gamma_div_batch<float>(const GammaBatchItem*, int, int64_t, TableArena&, hipStream_t);
+
+void launch_rowsum_reduce(const RowSumItem& item, hipStream_t st) {
+  hipLaunchKernelGGL(rowsum_reduce_kernel, dim3(9), dim3(256), 0, st, item);
+  SCA_HIP(hipGetLastError());
+}
+
+void launch_rowsum_reduce_batch(const RowSumItem* items, int n, TableArena& ta, hipStream_t st) {
+  if (n <= 0) return;
+  RowSumItem* d = nullptr;
+  RowSumItem* h = ta.add<RowSumItem>(n, &d);
+  for (int i = 0; i < n; ++i) h[i] = items[i];
+  ta.upload(h, n, st);
+  hipLaunchKernelGGL(rowsum_reduce_batch_kernel, dim3(9, (unsigned)n), dim3(256), 0, st, d);
+  SCA_HIP(hipGetLastError());
+}
 
 void launch_final_reduce(const double* partial, int nvec, int nb, double* out, hipStream_t st) {
   hipLaunchKernelGGL(final_reduce_kernel, dim3(nvec), dim3(256), 0, st, partial, nb, out);
