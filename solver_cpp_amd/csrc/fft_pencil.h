@@ -252,13 +252,35 @@ __device__ __forceinline__ void pencil_rows(int64_t pencil, int64_t set_rows, in
   if (ra >= nrows) ra = rb = nrows;
 }
 
+// x + (x of the lane DPP control CTRL selects), the halves moved separately
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+  return x + __hiloint2double(hi, lo);
+}
+// sum over each group of G consecutive lanes (G a power of two, groups
+// aligned), every lane of a group ending with the same bits: DPP exchanges
+// within rows of 16 lanes (quad xor 1, quad xor 2, half-row and row mirror),
+// xor shuffles across rows.  All lanes of the wave take part.
+template <int G>
+__device__ __forceinline__ double group_sum(double x) {
+  if constexpr (G >= 2) x = dpp_add<0xB1>(x);   // quad_perm [1,0,3,2]
+  if constexpr (G >= 4) x = dpp_add<0x4E>(x);   // quad_perm [2,3,0,1]
+  if constexpr (G >= 8) x = dpp_add<0x141>(x);  // row_half_mirror
+  if constexpr (G >= 16) x = dpp_add<0x140>(x); // row_mirror
+#pragma unroll
+  for (int off = 16; off < G; off <<= 1) x += __shfl_xor(x, off, G);
+  return x;
+}
+
 // x pass, real -> half complex, two real rows per complex pencil
 // (z = x_a + i x_b): X_a[k] = (Z[k] + conj Z[N-k]) / 2,
 // X_b[k] = (Z[k] - conj Z[N-k]) / (2 i).  rowsum (optional): the sum of each
 // input row in double precision, in a fixed order (each thread's points in
-// turn, then the pencil's TP lanes by an xor butterfly): the same for every
-// block shape, so that a field's sums can be taken from its forward
-// transform instead of another pass over it.
+// turn, then the pencil's TP lanes, group_sum): the same for every block
+// shape, so that a field's sums can be taken from its forward transform
+// instead of another pass over it.
 template <typename T, typename TI, int N>
 __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, cpx<T>* out, int64_t nrows,
                                                                  int64_t set_rows, const cpx<T>* tw_g,
@@ -284,11 +306,8 @@ __global__ void __launch_bounds__(MAX_PENCIL_BLOCK) x_r2c_kernel(const TI* in, c
     sb += (double)xb;
   }
   if (rowsum) {
-#pragma unroll
-    for (int off = S::TP / 2; off > 0; off >>= 1) {
-      sa += __shfl_xor(sa, off, S::TP);
-      sb += __shfl_xor(sb, off, S::TP);
-    }
+    sa = group_sum<S::TP>(sa);
+    sb = group_sum<S::TP>(sb);
     if (t == 0) {
       if (ra < nrows) rowsum[ra] = sa;
       if (rb < nrows) rowsum[rb] = sb;

@@ -717,11 +717,20 @@ __global__ void __launch_bounds__(256) final_reduce_batch_kernel(const BatchRedu
 }
 
 // component sums from x-pass row sums: block q of an item sums its rows
-// (each thread a stride-256 sequence in order, then the block tree)
+// (each thread a stride-256 sequence in order, its loads issued 16 at a
+// time ahead of the adds, then the block tree)
 __device__ __forceinline__ void rowsum_block(const RowSumItem& e, int q, double* red) {
   const double* r = e.rowsum + (int64_t)q * e.nrow;
   double s = 0.0;
-  for (int64_t t = threadIdx.x; t < e.nrow; t += 256) s += r[t];
+  constexpr int U = 16;
+  for (int64_t t0 = threadIdx.x; t0 < e.nrow; t0 += 256 * U) {
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = (t0 + 256 * u < e.nrow) ? r[t0 + 256 * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (t0 + 256 * u < e.nrow) s += x[u];
+  }
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
