@@ -280,6 +280,40 @@ def test_first_increment_homogeneous_forward(product, oracle):
     close(a.field("P"), b.field("P"), REL, "P")
 
 
+def test_state_read_between_forward_and_solve(product):
+    """f_materialpoint_forward (phase_mechanical.f90:992-1007) copies the
+    state into its start values.  A context above the captured-graph size
+    (80^3 > 64^3 cells) instead exchanges the current and start buffers of
+    Fp, S, xi and gamma and copies them back only for a reader that comes
+    before the next material-point update.  Read right after the forward
+    step, the fields are the previous increment's converged values, as after
+    the copy, and the next increment is bitwise the one of a context whose
+    fields were not read in between."""
+    cells = (80, 80, 80)
+    mat, ori, size = polycrystal(cells, 4, seed=11, ori_seed=12)
+    load = tension_x(t=10.0, N=20)
+    names = ["Fp", "S", "xi", "gamma"]
+    rs = [Rve(product, cells, size, mat, ori, **EXAMPLE_NUMERICS) for _ in range(2)]
+    try:
+        for r in rs:
+            r.init()
+            r.forward(False, False, 0.5, 1.0, 10.0, load)
+            r.calculate_solution()
+        before = {k: rs[0].field(k) for k in names}
+        assert not np.array_equal(before["gamma"], np.zeros_like(before["gamma"]))  # plastic
+        for r in rs:
+            r.forward(False, True, 0.5, 0.5, 9.5, load)
+        for k in names:  # only the first context reads
+            np.testing.assert_array_equal(rs[0].field(k), before[k], err_msg=k)
+        sa, sb = rs[0].calculate_solution(), rs[1].calculate_solution()
+        assert (sa.required_iterations, sa.residual_evaluations) == (sb.required_iterations, sb.residual_evaluations)
+        for k in ["F", "P", "Fp", "S", "xi", "gamma"]:
+            np.testing.assert_array_equal(rs[0].field(k), rs[1].field(k), err_msg=k)
+    finally:
+        for r in rs:
+            r.close()
+
+
 @pytest.mark.parametrize("cells", [(27, 18, 24), (20, 14, 22)])
 def test_fft_plan_modes_give_the_same_bits(product, cells):
     """fft_plan_mode 1 (launch shapes timed per grid, like FFTW_MEASURE) and 0
