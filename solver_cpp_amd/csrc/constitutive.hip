@@ -11,9 +11,11 @@
 //                                        phase_mechanical_plastic_phenopowerlaw.f90:298-483
 // with Fi = I and Li = 0 (no eigen kinematics are active).  The 9x9 Newton
 // Jacobian I - dLp/dS : dS/dFe : dFe/dLp is assembled directly from its
-// rank-12 structure: with G = Fe^T A and K_s = P_s : C,
+// rank-12 structure: with G = Fe^T A and K_s = P_s : C (cubic),
 //   J[(a,b),(p,l)] = delta + dt * sum_s c_s P_s(a,b) (G^T K_s)(p,l),
-// c_s = d(gamma_dot_s)/d(tau_s) * sign_s, instead of three 3x3x3x3 products.
+// c_s = d(gamma_dot_s)/d(tau_s) * sign_s, instead of three 3x3x3x3 products;
+// the sum over s is taken from the slip groups' Hadamard sums
+// (assemble_jacobian8), on the 8 traceless coordinates.
 #include "common.h"
 
 namespace sca {
@@ -38,12 +40,6 @@ __device__ constexpr int HTYPE[NSLIP][NSLIP] = {
     {3, 5, 5, 4, 6, 7, 1, 2, 2, 4, 7, 6}, {6, 4, 7, 5, 3, 5, 2, 1, 2, 7, 4, 6},
     {6, 7, 4, 7, 6, 4, 2, 2, 1, 5, 5, 3}, {4, 6, 7, 3, 5, 5, 4, 7, 6, 1, 2, 2},
     {5, 3, 5, 6, 4, 7, 7, 4, 6, 2, 1, 2}, {7, 6, 4, 6, 7, 4, 5, 5, 3, 2, 2, 1}};
-
-// K_s = P_s : C for the cubic stiffness (tr P_s = 0):
-// K_s(m,l) = v C44 (p_ml + p_lm) + v (C11 - C12 - 2 C44) delta_ml p_mm,
-// returned as coefficients of kA = v C44 and kB = v (C11 - C12 - 2 C44).
-__device__ __forceinline__ constexpr int kcoefA(int s, int m, int l) { return psg(s, m, l) + psg(s, l, m); }
-__device__ __forceinline__ constexpr int kcoefB(int s, int m, int l) { return m == l ? psg(s, m, m) : 0; }
 
 // Structure of the table used to share work between slip systems:
 // - the systems 3p, 3p+1, 3p+2 lie on the same plane (normal), p = 0..3;
@@ -485,13 +481,15 @@ __device__ __forceinline__ void load33_sel(bool sel, const double* a, const doub
 // the traceless coordinates (rows 0..7; columns 0 and 4 minus column 8), as
 // an 8x8 row-major matrix.
 //
-// With f_s = dt v c_s and K_s(m,l) = kappa(m,l) k_s(m,l) (kappa = 2 kA + kB on
-// the diagonal, kA off it), row r = 3i+j is
+// K_s = P_s : C for the cubic stiffness (tr P_s = 0) is K_s(m,l) =
+// kappa(m,l) k_s(m,l) with kappa = 2 kA + kB on the diagonal and kA off it,
+// kA = v C44, kB = v (C11 - C12 - 2 C44).  With f_s = dt v c_s, row r = 3i+j is
 //   J(r,(p,l)) = delta + sum_m kappa(m,l) G(m,p) Q_r(m,l),
 //   Q_r(m,l) = sum_s f_s P_s(i,j) k_s(m,l)   (symmetric in m,l),
 // and every Q_r(m,l) is a signed sum of one or two of the twelve Hadamard sums
 // of the direction groups' f_s (qhad): 24 additions for all the Q_r instead of
 // a rank-one update of the 8x8 matrix per slip system.
+
 // Row-independent part of the Jacobian: kappa G (or, LEAN, G alone, kappa
 // then scaling Q in every row: 18 registers fewer for 48 more products) and
 // the Hadamard sums
