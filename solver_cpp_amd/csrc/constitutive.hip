@@ -109,6 +109,25 @@ __device__ __forceinline__ void mm33(const double* A, const double* B, double* C
     for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
 }
 
+// Divisions without the compiler's range fix-ups, for operands far from the
+// range limits (slip resistances, the 3x3 determinants of deformation
+// gradients, the pivots of the Newton system, none of them near 2^+-1000):
+// 1/x as the hardware reciprocal refined by two Newton steps (as the
+// compiler's own division does before its fix-ups), and a / b as the rounded
+// product a (1/b) corrected once by its exact remainder a - b q (Markstein):
+// the correctly rounded quotient when 1/b is the correctly rounded
+// reciprocal, within one rounding of it otherwise
+__device__ __forceinline__ double rcp_refined(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ double div_rcp(double a, double b, double rb) {
+  const double q = a * rb;
+  return __builtin_fma(__builtin_fma(-q, b, a), rb, q);
+}
+
 // math_invert33 (math.f90:511-545); returns false if |det| <= 10*tiny
 __device__ __forceinline__ bool inv33(const double* A, double* R, double* detOut) {
   R[0] = A[4] * A[8] - A[5] * A[7];
@@ -122,8 +141,9 @@ __device__ __forceinline__ bool inv33(const double* A, double* R, double* detOut
   R[2] = A[1] * A[5] - A[2] * A[4];
   R[5] = -A[0] * A[5] + A[2] * A[3];
   R[8] = A[0] * A[4] - A[1] * A[3];
+  const double rd = rcp_refined(det);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) R[k] = R[k] / det;
+  for (int k = 0; k < 9; ++k) R[k] = div_rcp(R[k], det, rd);
   if (detOut) *detOut = det;
   return true;
 }
@@ -174,15 +194,6 @@ __device__ __forceinline__ double rqrt_refined(double x) {
     y = y * __builtin_fma(mq, y2 * y2, 1.25);
   }
   return y;
-}
-// 1/x for the slip resistances (positive, far from the range limits): the
-// hardware reciprocal refined by two Newton steps, as the compiler's own
-// division does before its range fix-ups
-__device__ __forceinline__ double rcp_refined(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-#pragma unroll
-  for (int k = 0; k < 2; ++k) r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
-  return r;
 }
 
 template <int E4>
@@ -618,7 +629,7 @@ __device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const 
     }
     const double piv = row[i];
     refused = refused || !(fabs(piv) > 0.0);
-    U[NR * i + i] = 1.0 / piv;
+    U[NR * i + i] = rcp_refined(piv);
 #pragma unroll
     for (int c2 = i + 1; c2 < NR; ++c2) U[NR * i + c2] = row[c2];
   }
@@ -794,8 +805,9 @@ __device__ __forceinline__ void store_integrated(const double (*hA)[64], const d
     store33(io.s->P, N, l, P);
   }
   const double d = pow(det33(Fpn), 1.0 / 3.0);
+  const double rd = rcp_refined(d);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) Fpn[k] = Fpn[k] / d;
+  for (int k = 0; k < 9; ++k) Fpn[k] = div_rcp(Fpn[k], d, rd);
   store33(io.s->Fp, N, l, Fpn);
 }
 
