@@ -599,7 +599,11 @@ __device__ __forceinline__ void assemble_jacobian8(const double* Fe, const doubl
 // formed (the same operations on every entry, in the same order, as the
 // column-by-column elimination), so only the factor rows are held.  Refuses
 // (returns false; b then holds no meaning) when a multiplier exceeds 1e6 in
-// magnitude or a pivot is zero: multipliers up to 1e6 cost at most a few of
+// magnitude or a pivot's magnitude is below 2^-1000 (zero and NaN included:
+// the pivot's reciprocal is the refined hardware one, so a pivot near the
+// bottom of the range goes to the pivoted pass and its exact division; the
+// pivots of this system, 1 plus rate terms, are never near the top of the
+// range): multipliers up to 1e6 cost at most a few of
 // the 16 digits of the Newton step, far below the Newton tolerance, and the
 // residual test of the next iteration uses the exact residual; only steeper
 // cases (about 1 % of points had pivots below 1/100 of their column in the
@@ -628,7 +632,7 @@ __device__ __forceinline__ bool assemble_solve8_nopivot(const double* Fe, const 
       b[i] -= l * b[k];
     }
     const double piv = row[i];
-    refused = refused || !(fabs(piv) > 0.0);
+    refused = refused || !(fabs(piv) >= 0x1p-1000);
     U[NR * i + i] = rcp_refined(piv);
 #pragma unroll
     for (int c2 = i + 1; c2 < NR; ++c2) U[NR * i + c2] = row[c2];
