@@ -71,7 +71,7 @@ void Batcher::leave() {
 }
 
 void Batcher::evaluate(Rve* rve, const double* F, void* r) {
-  Req q{rve, F, r};
+  Req q{rve, F, r, false, {}};
   std::unique_lock<std::mutex> lk(m_);
   if (waiting_.empty()) first_ = std::chrono::steady_clock::now();
   waiting_.push_back(&q);

@@ -152,12 +152,6 @@ __device__ __forceinline__ double det33(const double* m) {
   return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
 }
 
-__device__ __forceinline__ double norm9(const double* m) {
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) s += m[k] * m[k];
-  return sqrt(s);
-}
 
 // x^e for x >= 0 with e = e4/4: binary powering of the integer part times
 // sqrt / sqrt-sqrt for the quarter part; libm pow() when e4 < 0
@@ -435,10 +429,6 @@ __device__ __forceinline__ void load33(const double* base, int64_t N, int64_t c,
 #pragma unroll
   for (int q = 0; q < 9; ++q) m[rm(q)] = base[q * N + c];
 }
-__device__ __forceinline__ void store33(double* base, int64_t N, int64_t c, const double* m) {
-#pragma unroll
-  for (int q = 0; q < 9; ++q) base[q * N + c] = m[rm(q)];
-}
 
 // Plane addressing by a uniform plane pointer plus the lane's 32-bit byte
 // offset (scalar base + vector offset): one offset register serves every
@@ -657,10 +647,10 @@ struct PointIO {
 // integrateStress (phase_mechanical.f90:380-604); returns 0 (integrated),
 // 1 (broken) or, without PIVOT, 2 when the LU without row exchanges refused a
 // pivot (the point is then redone by the PIVOT instantiation, which solves
-// every Newton system with partial pivoting).  Uses the
-// state xi, updates S and Lp in registers and writes P, Fp, Fe of a
-// successful integration straight to memory (they are read again only as
-// results or as the start of a following substep).
+// every Newton system with partial pivoting).  Takes the reciprocal slip
+// resistances of the state and updates S and Lp in registers; Fe, P and Fp
+// of the integration the state loop keeps are formed and stored by
+// store_integrated.
 //
 // A = F invFp0 of the (sub)step and invFp0 (both in LDS, lane t) are formed
 // once per substep by the caller: they do not change during the state fixed
@@ -668,9 +658,7 @@ struct PointIO {
 template <int NE4, bool PIVOT>
 __device__ __forceinline__ int integrate_stress(const double (*hA)[64], const double (*invFp0)[64], int t, double dt,
                                                 const double* rxi, double* Lp, double* S, int& newton,
-                                                const PointIO& io, const MatConst& mc) {
-  const int64_t N = io.N;
-  const Lane l = io.l;
+                                                const MatConst& mc) {
   // A is read from LDS where it is used, not hoisted into the registers the
   // Newton system needs: the lane index passes through an empty asm, so every
   // call's nine ds_reads are new loads (issued together, one wait), not the
@@ -1025,7 +1013,7 @@ __device__ __forceinline__ void mp_point(const KernelArgs& a, const int64_t c, M
 #pragma unroll
           for (int k = 0; k < NSLIP; ++k) rxi[k] = rcp_refined(xi[k]);
           const int rc =
-              fp0_singular ? 1 : integrate_stress<NE4, PIVOT>(sh.A, sh.iFp0, t, sdt, rxi, Lp, S, newton, io, mc);
+              fp0_singular ? 1 : integrate_stress<NE4, PIVOT>(sh.A, sh.iFp0, t, sdt, rxi, Lp, S, newton, mc);
           refused = rc == 2;
           broken = rc != 0;
           if (broken) break;
