@@ -1408,7 +1408,9 @@ void Rve::forward(bool cutBack, bool guess, double dt, double dt_old, double t_r
     h::rot2_active(stp.rot_q, F_aim_dot, rotated);  // load -> field frame
     SCA_HIP(hipMemcpyAsync(small + SM_AV, rotated, 9 * sizeof(double), hipMemcpyHostToDevice, st));
     launch_rate(guess ? 1 : 0, F_last_inc, X, dt_old, small + SM_AV, F_dot, N, st);
-    SCA_HIP(hipMemcpyAsync(F_last_inc, X, 9 * N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // F_last_inc <- X by exchanging the buffers: X is written whole below
+    // (launch_fwd_field, from F_last_inc), so its old contents are not needed
+    std::swap(F_last_inc, X);
     rate_uniform = !guess;
     std::memcpy(rate_value, rotated, sizeof rate_value);
     last_uniform = x_uniform;
